@@ -1,0 +1,160 @@
+/*
+ * gepi_cli.cpp -- the `gepicount` command line (replaces src/gEpiCount.cpp).
+ *
+ * Flags and defaults follow getParams/printHelp (src/gEpiCount.cpp:98-187):
+ *   -i/--input-root BASE, -f/--fisher-pvals, -G/--gpu-num N, -N/--topmost M,
+ *   -t/--alternate-grid, -D/--print-diagonal, --block-sizes X Y,
+ *   --max-grid-area N, --shared-memory, --miss-snps F, --miss-samples F.
+ * -g/--gold is refused: the CPU gold scan is the test oracle (oracle/), and
+ * this tool only runs on the GPU.  Output streams follow main() (:47-96): the
+ * informational lines go to stdout, progress and the ranked report
+ * (printInteractions, src/misc/print_misc.h:56-78) to stderr.
+ * Exit codes: 0 ok, 3 bad arguments (as the reference), 1 any other error.
+ */
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gepicount.h"
+
+namespace {
+
+const char *PROGNAME = "gEpiCount";
+
+void print_help() {
+    fprintf(stderr,
+            "%s -i <input-root> [options]\n\n"
+            "   -i | --input-root      BASE Use BASE as full path and filename for .bed,.bim,.fam\n\n"
+            "   -f | --fisher-pvals         Computes Fisher P-value instead of Entropy [no]\n"
+            "   -G | --gpu-num         N    Use GPU number N  [0]\n"
+            "   -t | --alternate-grid       Accepted for compatibility (no effect)\n"
+            "   -D | --print-diagonal       Print a sample of the ranking (stride 100000) [no]\n"
+            "   -N | --topmost         M    Maintain a list of M results  [%d]\n\n"
+            "        --block-sizes     N M  Accepted for compatibility; N*M must be a multiple of 32 [16,16]\n"
+            "        --max-grid-area   N    Cap evaluations per kernel launch at N x 1024 x 1024 [auto]\n"
+            "        --shared-memory        Accepted for compatibility (tiles always use LDS)\n"
+            "        --count-path      P    popcount | mfma | auto  [auto]\n"
+            "        --miss-snps       F    Max missing values fraction for SNPs [0.05]\n"
+            "        --miss-samples    F    Max missing values fraction for samples  [0.05]\n"
+            "   -g | --gold                 Not available: the CPU gold scan is oracle/gepicount_gold\n\n",
+            PROGNAME, GEPI_DEFAULT_TOPMOST);
+}
+
+bool need(int j, int argc, int extra) { return j + extra < argc; }
+
+} // namespace
+
+int main(int argc, char **argv) {
+    fprintf(stderr, "%s, version: %s\n\n", PROGNAME, gepi_version());
+    gepi_params par;
+    gepi_params_init(&par);
+    std::string root;
+    float miss_snps = 0.05f, miss_samp = 0.05f;
+    bool ok = true, diag = false;
+    for (int j = 1; j < argc; ++j) {
+        std::string a(argv[j]);
+        if (a == "-g" || a == "--gold") {
+            fprintf(stderr, "The CPU gold scan is not part of this tool; use oracle/gepicount_gold.\n");
+            ok = false;
+        } else if (a == "-t" || a == "--alternate-grid" || a == "--shared-memory") {
+        } else if ((a == "-G" || a == "--gpu-num") && need(j, argc, 1)) {
+            par.gpu_num = atoi(argv[++j]);
+        } else if (a == "-D" || a == "--print-diagonal") {
+            diag = true;
+            par.intermediate = 1;
+        } else if (a == "-f" || a == "--fisher-pvals") {
+            par.ffun = GEPI_FFUNC_FISHERPV;
+        } else if ((a == "-i" || a == "--input-root") && need(j, argc, 1)) {
+            root = argv[++j];
+        } else if ((a == "-N" || a == "--topmost") && need(j, argc, 1)) {
+            par.topmost = atoll(argv[++j]);
+        } else if (a == "--block-sizes" && need(j, argc, 2)) {
+            par.blk_x = atoi(argv[++j]);
+            par.blk_y = atoi(argv[++j]);
+        } else if (a == "--max-grid-area" && need(j, argc, 1)) {
+            par.max_grid_area = atoi(argv[++j]);
+        } else if (a == "--count-path" && need(j, argc, 1)) {
+            std::string v(argv[++j]);
+            if (v == "popcount") par.count_path = 1;
+            else if (v == "mfma") par.count_path = 2;
+            else if (v == "auto") par.count_path = 0;
+            else {
+                fprintf(stderr, "Unknown count path: %s\n", v.c_str());
+                ok = false;
+            }
+        } else if (a == "--miss-snps" && need(j, argc, 1)) {
+            miss_snps = (float)atof(argv[++j]);
+        } else if (a == "--miss-samples" && need(j, argc, 1)) {
+            miss_samp = (float)atof(argv[++j]);
+        } else {
+            fprintf(stderr, "Unknown option: %s\n", a.c_str());
+            ok = false;
+        }
+    }
+    if ((par.blk_x * par.blk_y) % 32 != 0) {
+        fprintf(stderr, "Block size not multiple of war size (32)\n");
+        ok = false;
+    }
+    if (root.empty()) {
+        fprintf(stderr, "Insufficient arguments\n");
+        ok = false;
+    }
+    if (par.topmost < 0) {
+        fprintf(stderr, "topmost must be >= 0\n");
+        ok = false;
+    }
+    if (!ok) {
+        print_help();
+        return 3;
+    }
+
+    gepi_plink *pr = nullptr;
+    if (gepi_plink_open(root.c_str(), &pr) != GEPI_OK) {
+        printf("Error in %s\n", root.c_str());
+        fprintf(stderr, "%s\n", gepi_last_error());
+        return 1;
+    }
+    printf("\nElement Size: %d\n", 8);
+    size_t dsz = (size_t)8 * pr->words * pr->n_rows;
+    printf("Size of each Data Matrix: %zu (%zuMB)\n", dsz, (dsz + (1 << 20) - 1) >> 20);
+    gepi_plink_check_quality(pr, miss_snps, miss_samp, nullptr, nullptr);
+    if (!pr->mask_ctrl || !pr->mask_case) {
+        fprintf(stderr, "Error in class mask data\n");
+        gepi_plink_close(pr);
+        return 1;
+    }
+    size_t tsz = (size_t)par.topmost * sizeof(gepi_result);
+    printf("Size of array of %lld topmost interaction results: %zu (%zuMB)\n",
+           (long long)par.topmost, tsz, (tsz + (1 << 20) - 1) >> 20);
+    fflush(stdout);
+
+    std::vector<gepi_result> tops((size_t)(par.topmost > 0 ? par.topmost : 1));
+    int64_t n = 0;
+    gepi_scan_stats st;
+    gepi_panel *panel = nullptr;
+    int rc = gepi_panel_create_bed(pr->bed_payload, pr->n_rows / 2, pr->n_samples, par.gpu_num,
+                                   &panel);
+    if (rc == GEPI_OK) {
+        rc = gepi_scan(panel, pr->mask_ctrl, pr->mask_case, &par, tops.data(), &n, &st);
+        gepi_panel_destroy(panel);
+    }
+    if (rc != GEPI_OK) {
+        fprintf(stderr, "E[%d]: scan failed (%d): %s\n", par.gpu_num, rc, gepi_last_error());
+        gepi_plink_close(pr);
+        return 1;
+    }
+    fprintf(stderr,
+            "K1: objective %s, %lld evaluations, %d+%d class samples, count path %d, %d launches\n"
+            "K1: Time: scan %.3f ms (count kernels %.3f ms, layout %.3f ms, top-k %.3f ms), "
+            "exact evaluations %lld\n",
+            par.ffun == GEPI_FFUNC_ENTROPY ? "ENTROPY" : "FISHER PV", (long long)st.evals,
+            st.n_class0, st.n_class1, st.count_path, st.bands, st.ms_scan, st.ms_count_kernels,
+            st.ms_prepare, st.ms_select, (long long)st.exact_evals);
+    if (diag) gepi_print_interactions(pr, tops.data(), n, 100000, 5);
+    gepi_print_interactions(pr, tops.data(), n, 1, 0);
+    gepi_plink_close(pr);
+    printf("Cleaned\n");
+    return 0;
+}

@@ -1,0 +1,176 @@
+/*
+ * gepicount.h -- C ABI of the MI355X gEpiCount scan engine (libgepicount.so).
+ *
+ * Plain C types only: pointers, sizes, status codes.  The library owns all
+ * device memory; callers own every host buffer they pass in.  Every entry
+ * point returns GEPI_OK (0) or a negative gepi_status and never throws;
+ * gepi_last_error() gives the message of the last failure on the calling
+ * thread.  There is no CPU fallback: without a usable gfx950 GPU the scan
+ * entry points fail with GEPI_ERR_DEVICE.
+ */
+#ifndef GEPICOUNT_H
+#define GEPICOUNT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GEPI_NUM_FUNCS 5
+#define GEPI_NUM_CLASSES 2
+/* Upper bound on samples per class layout; the count kernels keep all table
+ * arithmetic in 32-bit integers up to this size.  The reference caps samples
+ * at 8192 (CONST_MASK_MAX_SIZE, inc/gEpiCount.h:51; kernel1.cu:329-333).    */
+#define GEPI_MAX_SAMPLES 46340
+#define GEPI_DEFAULT_TOPMOST 1000000 /* src/gEpiCount.cpp:34 */
+
+/* Boolean functions: enum BinaryFunction, inc/gEpiCount.h:58-66.  The names
+ * follow the reference: NAND = ~A & B, ANDN = A & ~B, NANDN = ~(A | B).      */
+typedef enum gepi_bfunc {
+    GEPI_BFUNC_AND = 0,
+    GEPI_BFUNC_NAND = 1,
+    GEPI_BFUNC_ANDN = 2,
+    GEPI_BFUNC_NANDN = 3,
+    GEPI_BFUNC_XOR = 4
+} gepi_bfunc;
+
+/* Objectives: enum FitnessFunction, inc/gEpiCount.h:68-73. */
+typedef enum gepi_ffunc { GEPI_FFUNC_ENTROPY = 0, GEPI_FFUNC_FISHERPV = 1 } gepi_ffunc;
+
+typedef enum gepi_status {
+    GEPI_OK = 0,
+    GEPI_ERR_ARG = -1,      /* invalid argument                               */
+    GEPI_ERR_LAYOUT = -2,   /* rows not paired as (allele 1, allele 2) sharing valid bits */
+    GEPI_ERR_LIMIT = -3,    /* size beyond a documented limit                 */
+    GEPI_ERR_NOMEM = -4,    /* device or host allocation failed               */
+    GEPI_ERR_DEVICE = -5,   /* no usable GPU / HIP runtime error              */
+    GEPI_ERR_IO = -6,       /* file missing or unreadable                     */
+    GEPI_ERR_FORMAT = -7,   /* malformed .bed/.bim/.fam                       */
+    GEPI_ERR_NOCLASS = -8,  /* phenotype class 1 or 2 absent                  */
+    GEPI_ERR_INTERNAL = -9
+} gepi_status;
+
+/* One ranked interaction.  Replaces InteractionResult (inc/gEpiCount.h:75-82):
+ * sA >= sB are allele-row indices, (w, x, y, z) is count_type4 n, obj the
+ * objective (entropy delta -I(F;C) in bits, or log10 Fisher p), fun the
+ * gepi_bfunc.  48 bytes, 8-byte aligned.                                     */
+typedef struct gepi_result {
+    int64_t sA;
+    int64_t sB;
+    int32_t w, x, y, z;
+    double obj;
+    int32_t fun;
+    int32_t reserved;
+} gepi_result;
+
+/* Scan parameters.  Replaces struct _paramP1 (inc/gEpiCount.h:116-132);
+ * the file name and QA thresholds stay with the caller.                      */
+typedef struct gepi_params {
+    int32_t gpu_num;       /* -G: HIP device ordinal                           */
+    int32_t ffun;          /* -f: gepi_ffunc                                   */
+    int64_t topmost;       /* -N: results to keep                              */
+    int32_t count_path;    /* 0 = fastest available, 1 = popcount, 2 = MFMA    */
+    int32_t intermediate;  /* -D: caller prints intermediate results (CLI)     */
+    int32_t blk_x, blk_y;  /* --block-sizes: accepted for compatibility        */
+    int32_t max_grid_area; /* --max-grid-area: cap on evals per launch, in Mi  */
+    int32_t reserved;
+} gepi_params;
+
+/* Per-scan measurements filled by the scan entry points (may be NULL). */
+typedef struct gepi_scan_stats {
+    double ms_scan;          /* host wall time of the scan, panel resident     */
+    double ms_count_kernels; /* sum of count-kernel durations (HIP events)     */
+    double ms_prepare;       /* class layout build (gather kernel)             */
+    double ms_select;        /* top-k maintenance kernels + final extraction   */
+    int64_t evals;           /* row pairs x 5 functions                        */
+    int64_t exact_evals;     /* evaluations admitted to the exact score        */
+    int32_t bands;           /* count-kernel launches                          */
+    int32_t count_path;      /* path actually used                             */
+    int32_t n_class0, n_class1;
+} gepi_scan_stats;
+
+/* Defaults of src/gEpiCount.cpp:34-39 and :100-115. */
+void gepi_params_init(gepi_params *par);
+
+/* Plane-level scan on device `par->gpu_num`.  Mirrors
+ *   void process_gpu_1(PlinkReader<ui8> *pr, InteractionResult *tops,
+ *                      const struct _paramP1 &par)   (src/proc_gpu.h:134,
+ *                                                     src/kernels/kernel1.cu:508-511)
+ * bit_data/valid_data are the reader's BitData()/ValidData() [n_rows][words]
+ * with words = ceil(n_samples/64) (src/reader/PlinkReader.h:199-225); rows
+ * 2i and 2i+1 are the two alleles of one variant and share valid bits.
+ * mask_ctrl/mask_case are classData(1)/classData(2)->mask() [words].
+ * tops receives min(par->topmost, 5 * n_rows*(n_rows+1)/2) results sorted by
+ * (obj, sA, sB, fun) ascending; their number goes to *n_tops.               */
+int gepi_process_gpu_1(const uint64_t *bit_data, const uint64_t *valid_data,
+                       const uint64_t *mask_ctrl, const uint64_t *mask_case, int64_t n_rows,
+                       int32_t n_samples, const gepi_params *par, gepi_result *tops,
+                       int64_t *n_tops, gepi_scan_stats *stats);
+
+/* ---- resident panels (serving / repeated scans) --------------------------
+ * A panel keeps the packed allele planes on one device across scans; each
+ * scan supplies its own class masks.  No counterpart in the reference, which
+ * re-uploads the panel per call (src/kernels/setup1.h:72-102).               */
+typedef struct gepi_panel gepi_panel;
+
+/* From reader planes (same layout and pairing rule as gepi_process_gpu_1). */
+int gepi_panel_create(const uint64_t *bit_data, const uint64_t *valid_data, int64_t n_rows,
+                      int32_t n_samples, int32_t device, gepi_panel **out);
+/* From raw SNP-major .bed genotype bytes (the file after its 3-byte header):
+ * n_variants blocks of ceil(n_samples/4) bytes, decoded on the device with the
+ * rules of src/reader/BinPlinkReader.h:171-201.                              */
+int gepi_panel_create_bed(const uint8_t *bed_payload, int64_t n_variants, int32_t n_samples,
+                          int32_t device, gepi_panel **out);
+int gepi_panel_destroy(gepi_panel *panel);
+/* Scan a resident panel with the given class masks ([words] each). */
+int gepi_scan(gepi_panel *panel, const uint64_t *mask_ctrl, const uint64_t *mask_case,
+              const gepi_params *par, gepi_result *tops, int64_t *n_tops,
+              gepi_scan_stats *stats);
+/* Reads the panel's planes back in reader layout (testing aid). */
+int gepi_panel_download(gepi_panel *panel, uint64_t *bit_data, uint64_t *valid_data);
+
+/* ---- host-side PLINK reader and report ---------------------------------- */
+/* Replaces BinPlinkReader<ui8> (src/reader/BinPlinkReader.h, PlinkReader.h).
+ * Fields are read-only for the caller; release with gepi_plink_close.        */
+typedef struct gepi_plink {
+    int64_t n_rows;       /* numSNPs(): 2 x .bim lines                        */
+    int32_t n_samples;    /* numSamples()                                     */
+    int32_t words;        /* elementsSNP()                                    */
+    uint64_t *bit_data;   /* BitData()   [n_rows][words]                      */
+    uint64_t *valid_data; /* ValidData() [n_rows][words]                      */
+    uint64_t *mask_ctrl;  /* classData(1)->mask(), NULL if no sample has phenotype 1 */
+    uint64_t *mask_case;  /* classData(2)->mask(), NULL if no sample has phenotype 2 */
+    int32_t *pheno;       /* per sample phenotype (Sample::Class())           */
+    char **sample_name;   /* per sample (Sample::Name(): first .fam token)    */
+    char **snp_id;        /* per row (SNP::Id())                              */
+    char *allele;         /* per row (SNP::Alleles())                         */
+    int32_t *chr, *pos;   /* per row                                          */
+    int64_t total_missing;
+    uint8_t *bed_payload; /* raw genotype bytes after the header              */
+} gepi_plink;
+
+/* checkConsistency() + readData(): 0 or negative gepi_status. */
+int gepi_plink_open(const char *root, gepi_plink **out);
+void gepi_plink_close(gepi_plink *p);
+/* checkQuality(remove=false, ...) (PlinkReader.h:237-335): reports samples and
+ * rows above the missing fractions on stderr; counts go to the out args.    */
+int gepi_plink_check_quality(const gepi_plink *p, float max_miss_snps, float max_miss_samples,
+                             int32_t *n_bad_samples, int32_t *n_bad_rows);
+/* printInteractions(pr, tops, numT, stride, context) (src/misc/print_misc.h:56-67). */
+int gepi_print_interactions(const gepi_plink *p, const gepi_result *tops, int64_t n,
+                            int32_t stride, int32_t context);
+/* fisher_pval (src/misc/fisher_p.c:52-76, declared inc/gEpiCount.h:137-139):
+ * log10 two-sided Fisher p of the table (n11, n12; n21, n22), as float.      */
+float gepi_fisher_pval(int32_t n11, int32_t n12, int32_t n21, int32_t n22);
+
+/* ---- misc ---------------------------------------------------------------- */
+const char *gepi_last_error(void);
+const char *gepi_version(void);
+/* Number of visible HIP devices (0 when none); status as return value. */
+int gepi_device_count(int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEPICOUNT_H */

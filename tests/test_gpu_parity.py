@@ -1,0 +1,186 @@
+"""GPU parity: the HIP scan (through the C ABI) against the CPU oracle.
+
+Counts, objective bits and top-k order must be identical.  Sizes are chosen so
+that the oracle finishes in seconds; workloads at full size are covered by
+size-independent properties in test_gpu_properties.py.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gepicount_amd as ge
+from gepicount_amd import synth
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DATA = os.path.join(ROOT, "tests", "golden", "data")
+FIELDS = ("sA", "sB", "fun", "w", "x", "y", "z", "obj")
+
+
+def assert_same(got, ref, label=""):
+    assert len(got) == len(ref), (label, len(got), len(ref))
+    for k in FIELDS:
+        if not np.array_equal(got[k], ref[k]):
+            bad = int(np.flatnonzero(got[k] != ref[k])[0])
+            raise AssertionError("%s: field %s differs first at rank %d: got %r ref %r" % (
+                label, k, bad, got[bad], ref[bad]))
+
+
+def random_panel(seed, n_var, n, miss, cls_p=(0.5, 0.45, 0.05)):
+    rng = np.random.default_rng(seed)
+    codes = orc.random_codes(rng, n_var, n, miss)
+    cls = rng.choice([0, 1, -1], size=n, p=list(cls_p))
+    bits, valid = orc.planes_from_codes(codes)
+    m0, m1 = orc.masks_from_classes(cls)
+    return bits, valid, m0, m1
+
+
+@pytest.mark.parametrize("seed,n_var,n,miss", [
+    (1, 1, 7, 0.0), (2, 5, 64, 0.0), (3, 33, 65, 0.05), (4, 40, 200, 0.1), (5, 70, 129, 0.02),
+    (6, 32, 300, 0.0)])
+def test_all_evaluations_entropy(seed, n_var, n, miss):
+    print("case seed=%d n_var=%d n=%d miss=%g" % (seed, n_var, n, miss), flush=True)
+    bits, valid, m0, m1 = random_panel(seed, n_var, n, miss)
+    R = 2 * n_var
+    total = 5 * R * (R + 1) // 2
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=total)
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), total)
+    assert_same(got, ref, "all-evals")
+    assert st["evals"] == total
+
+
+@pytest.mark.parametrize("seed,n_var,n,miss,k", [
+    (11, 150, 400, 0.02, 50), (12, 200, 700, 0.0, 500), (13, 97, 1000, 0.05, 7),
+    (14, 260, 256, 0.01, 3000)])
+def test_topk_entropy_threshold_active(seed, n_var, n, miss, k):
+    print("case seed=%d n_var=%d n=%d k=%d" % (seed, n_var, n, k), flush=True)
+    bits, valid, m0, m1 = random_panel(seed, n_var, n, miss)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k)
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), k)
+    assert_same(got, ref, "topk")
+    # the filter must have pruned most evaluations
+    assert st["exact_evals"] < st["evals"]
+
+
+@pytest.mark.parametrize("seed,n_var,n,miss,k", [(21, 12, 90, 0.05, None), (22, 60, 300, 0.02, 40)])
+def test_fisher_objective(seed, n_var, n, miss, k):
+    print("case fisher seed=%d n_var=%d n=%d" % (seed, n_var, n), flush=True)
+    bits, valid, m0, m1 = random_panel(seed, n_var, n, miss)
+    R = 2 * n_var
+    kk = k or 5 * R * (R + 1) // 2
+    got, _ = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=kk, ffun=ge.GEPI_FFUNC_FISHERPV)
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), kk, ffun=1)
+    assert_same(got, ref, "fisher")
+
+
+def test_kat_test_fixture_top5():
+    d = ge.read_plink(os.path.join(DATA, "test"))
+    got, _ = ge.process_gpu_1(d.bit_data, d.valid_data, d.mask_ctrl, d.mask_case, d.n_samples,
+                              topmost=5)
+    assert [(int(r["sA"]), int(r["sB"]), int(r["fun"])) for r in got] == \
+        [(2, 0, 0), (2, 1, 1), (2, 2, 0), (2, 2, 3), (3, 0, 2)]
+    assert np.all(got["obj"] == got["obj"][0])
+    assert abs(got["obj"][0] + 0.4591479170272448) < 1e-15
+
+
+@pytest.mark.parametrize("ffun", [0, 1])
+def test_thinhap1_top1000(ffun):
+    root = os.path.join(DATA, "thinhap1")
+    d = ge.read_plink(root)
+    with ge.Panel.from_bed(d.bed_payload, d.n_rows // 2, d.n_samples) as p:
+        got, _ = p.scan(d.mask_ctrl, d.mask_case, topmost=1000, ffun=ffun)
+    ref = orc.gold_scan(orc.read_plink(root), 1000, ffun=ffun)
+    assert_same(got, ref, "thinhap1")
+
+
+def test_small_workload_top100_both_objectives():
+    p = synth.make_panel("small")
+    m0, m1 = synth.masks_from_pheno(p["pheno"])
+    with ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"]) as panel:
+        bits, valid = panel.download()
+        op = orc.Panel(bits, valid, m0, m1, p["n_samples"])
+        for ffun in (0, 1):
+            got, _ = panel.scan(m0, m1, topmost=100, ffun=ffun)
+            ref = orc.gold_scan(op, 100, ffun=ffun)
+            assert_same(got, ref, "small ffun=%d" % ffun)
+
+
+def test_medium_subset_top1000():
+    """Exact match vs the gold scan on a 2,000-variant subset of the medium workload."""
+    p = synth.make_panel("medium", n_var=2000)
+    m0, m1 = synth.masks_from_pheno(p["pheno"])
+    with ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"]) as panel:
+        got, st = panel.scan(m0, m1, topmost=1000)
+        bits, valid = panel.download()
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, p["n_samples"]), 1000)
+    assert_same(got, ref, "medium-2000")
+    # the planted XOR pairs dominate the ranking
+    top_pairs = {(int(r["sA"]) // 2, int(r["sB"]) // 2) for r in got[:50]}
+    planted = {(max(a, b), min(a, b)) for a, b in p["planted"]}
+    assert len(top_pairs & planted) >= 3
+
+
+def test_bed_decode_equals_host_planes():
+    d = ge.read_plink(os.path.join(DATA, "thinhap1"))
+    with ge.Panel.from_bed(d.bed_payload, d.n_rows // 2, d.n_samples) as p:
+        bits, valid = p.download()
+    assert np.array_equal(bits, d.bit_data & d.valid_data)
+    assert np.array_equal(valid, d.valid_data)
+
+
+def test_resident_panel_rescans_new_phenotypes():
+    bits, valid, m0, m1 = random_panel(31, 90, 500, 0.02)
+    with ge.Panel.from_planes(bits, valid, 500) as panel:
+        rng = np.random.default_rng(5)
+        for i in range(3):
+            cls = rng.choice([0, 1, -1], size=500, p=[0.4, 0.5, 0.1])
+            a, b = orc.masks_from_classes(cls)
+            got, _ = panel.scan(a, b, topmost=200)
+            ref = orc.gold_scan(orc.Panel(bits, valid, a, b, 500), 200)
+            assert_same(got, ref, "rescan %d" % i)
+
+
+def test_overlapping_and_empty_masks():
+    bits, valid, m0, m1 = random_panel(41, 20, 130, 0.0)
+    both = m0 | m1
+    for a, b in ((both, m1), (m0, np.zeros_like(m1)), (np.zeros_like(m0), np.zeros_like(m1))):
+        got, _ = ge.process_gpu_1(bits, valid, a, b, 130, topmost=400)
+        ref = orc.gold_scan(orc.Panel(bits, valid, a, b, 130), 400)
+        assert_same(got, ref, "masks")
+
+
+def test_overflow_recovery_path(monkeypatch):
+    """A tiny candidate buffer forces the band to overflow and be redone."""
+    bits, valid, m0, m1 = random_panel(51, 300, 300, 0.01)
+    monkeypatch.setenv("GEPI_CAND_CAP", "1")
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, 300, topmost=100)
+    monkeypatch.delenv("GEPI_CAND_CAP")
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 300), 100)
+    assert_same(got, ref, "overflow")
+
+
+def test_topk_zero_and_more_than_all():
+    bits, valid, m0, m1 = random_panel(61, 4, 70, 0.0)
+    got, _ = ge.process_gpu_1(bits, valid, m0, m1, 70, topmost=0)
+    assert len(got) == 0
+    got, _ = ge.process_gpu_1(bits, valid, m0, m1, 70, topmost=10**6)
+    assert len(got) == 5 * 8 * 9 // 2
+
+
+def test_cli_report_equals_oracle_report():
+    root = os.path.join(DATA, "thinhap1")
+    exe = os.path.join(ROOT, "gepicount_amd", "gepicount")
+    r = subprocess.run([exe, "-i", root, "-N", "100"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = r.stderr.splitlines()
+    start = lines.index("Rank:\tidxA[id allele chr:pos]\tFunc\tidxB[id allele chr:pos]\tObj\tFPv\t"
+                        "(c00,c01,c10,c11)")
+    report = lines[start - 1:start + 101]
+    gold = subprocess.run([os.path.join(ROOT, "oracle", "gepicount_gold"), "-i", root, "-N", "100"],
+                          capture_output=True, text=True, timeout=120)
+    assert report == gold.stderr.splitlines()[:102]
+    assert "Cleaned" in r.stdout
