@@ -1,0 +1,169 @@
+"""Benchmark: all-pairs SNP x Boolean-function scan throughput on one MI355X.
+
+A step is one complete scan of a panel already resident in HBM: class layout,
+every count-kernel band, top-k maintenance and the final extraction of the
+top-k to the host.  The workload is the 100,000-variant x 8,192-sample panel
+(2,458 cases, 2% missing, top-10,000, entropy objective).
+
+Prints one JSON line (rank 0, N = 1):
+  value       evals/s = 5 * R(R+1)/2 / mean step seconds, R = 2 x variants
+  roofline    the count kernel: algorithmic lane-ops per launch / mean launch
+              time (HIP events on the scan stream) against the gfx950 VALU peak
+  cpu_baseline  the oracle's OpenMP gold scan on a bounded leading subset of
+              the same panel, on this host's cores
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SNP-pair x Boolean-fn evals/s at 100k SNPs x 8k samples; p50/p99 scan latency"
+# gfx950: 256 CUs x 64 lanes x 2.4 GHz of 32-bit VALU lane operations
+VALU_PEAK_OPS = 256 * 64 * 2.4e9
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="large", choices=["small", "medium", "large"])
+    ap.add_argument("--n-var", type=int, default=None, help="override variant count")
+    ap.add_argument("--topk", type=int, default=10000)
+    ap.add_argument("--ffun", type=int, default=0)
+    ap.add_argument("--count-path", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the cpu_baseline sample (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (counter scale)")
+    return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size, so that HBM byte counters
+    (FETCH_SIZE / WRITE_SIZE) can be scaled: it reads and writes mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_each_way": mib << 20}))
+
+
+def class_words(n):
+    return ((n + 127) // 128) * 4
+
+
+def cpu_baseline(panel, m0, m1, n_samples, target_s, topk, ffun):
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    bits, valid = panel.download()
+
+    def run(v):
+        sub = orc.Panel(bits[:2 * v], valid[:2 * v], m0, m1, n_samples)
+        t = time.perf_counter()
+        orc.gold_scan(sub, topk, ffun=ffun, nthreads=threads)
+        dt = time.perf_counter() - t
+        R = 2 * v
+        return 5 * R * (R + 1) // 2, dt
+
+    v = 400
+    ev, dt = run(v)
+    # work grows with v^2: pick the subset that takes about target_s
+    v2 = int(v * (target_s / max(dt, 1e-3)) ** 0.5)
+    v2 = max(v, min(v2, bits.shape[0] // 2))
+    ev, dt = run(v2)
+    return {"value": ev / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": "oracle gold scan (OpenMP, exact scores, top-%d) of the first %d variants "
+                      "(%d evals) of the same panel, %.1f s" % (topk, v2, ev, dt)}
+
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import gepicount_amd as ge
+    from gepicount_amd import synth
+
+    t = time.perf_counter()
+    p = synth.make_panel(args.workload, n_var=args.n_var)
+    gen_s = time.perf_counter() - t
+    n_var, n = p["n_var"], p["n_samples"]
+    m0, m1 = synth.masks_from_pheno(p["pheno"])
+    n0, n1 = int(np.sum(p["pheno"] == 1)), int(np.sum(p["pheno"] == 2))
+    R = 2 * n_var
+    evals = 5 * R * (R + 1) // 2
+    panel = ge.Panel.from_bed(p["payload"], n_var, n)
+
+    for _ in range(args.warmup):
+        panel.scan(m0, m1, topmost=args.topk, ffun=args.ffun, count_path=args.count_path)
+    times, stats = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        res, st = panel.scan(m0, m1, topmost=args.topk, ffun=args.ffun,
+                             count_path=args.count_path)
+        times.append(time.perf_counter() - ts)
+        stats.append(st)
+    total = time.perf_counter() - t0
+    ms_per_step = 1000.0 * total / args.steps
+
+    # count kernel roofline: popcount lane-ops the kernel's algorithm needs
+    # = 9 products per variant pair x 2 ops (and, bcnt) x 32-bit words of data
+    var_pairs = n_var * (n_var + 1) // 2
+    words = (n0 + n1) / 32.0
+    ops_total = var_pairs * 9 * 2 * words
+    k_ms = statistics.mean(s["ms_count_kernels"] for s in stats)
+    launches = statistics.mean(s["bands"] for s in stats)
+    achieved = ops_total / (k_ms / 1000.0)
+    roof = {"bound": "valu", "kernel": "k_count_popc", "achieved": achieved / 1e12,
+            "peak": VALU_PEAK_OPS / 1e12, "unit": "Tops/s", "frac": achieved / VALU_PEAK_OPS,
+            "traffic": None,
+            "ops_per_launch": ops_total / launches, "launch_ms": k_ms / launches,
+            "launches": launches,
+            "reference_unit_frac": (R * (R + 1) / 2) * (n / 4.0) / (k_ms / 1000.0) / VALU_PEAK_OPS}
+    prof = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (roof["kernel"], args.workload))
+    if os.path.exists(prof):
+        roof["traffic"] = json.load(open(prof)).get("bytes_per_launch")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(panel, m0, m1, n, args.cpu_seconds, args.topk, args.ffun)
+    panel.close()
+
+    lat = sorted(times)
+    line = {
+        "metric": METRIC, "value": evals / statistics.mean(times), "unit": "evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "%s: %d variants x %d samples (%d cases / %d controls), top-%d, %s"
+                               % (args.workload, n_var, n, n1, n0, args.topk,
+                                  "entropy" if args.ffun == 0 else "fisher"),
+                   "variants": n_var, "samples": n, "row_pairs": R * (R + 1) // 2,
+                   "evals": evals, "topk": args.topk, "count_path": stats[-1]["count_path"]},
+        "p50_ms": 1000.0 * lat[len(lat) // 2], "p99_ms": 1000.0 * lat[min(len(lat) - 1,
+                                                                           int(0.99 * len(lat)))],
+        "variant_pairs_per_s": (n_var * (n_var - 1) // 2) / statistics.mean(times),
+        "exact_evals": stats[-1]["exact_evals"], "gen_s": gen_s,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -343,7 +343,9 @@ __global__ void __launch_bounds__(COUNT_THREADS) k_count_popc(CountArgs g) {
                             const int32_t q = xx * (m - xx);
                             const int32_t d = w * z - x * y;
                             const float fd = (float)d;
-                            pass = fd * fd >= kyz * (float)q * lo_margin;
+                            /* d == 0 is an independent table (I = 0): it only
+                             * qualifies while the threshold admits everything */
+                            pass = (kf == 0.0f) || (d != 0 && fd * fd >= kyz * (float)q * lo_margin);
                         } else {
                             pass = true;
                             if (thr_s < 0.0) {
