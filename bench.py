@@ -7,8 +7,9 @@ top-k to the host.  The workload is the 100,000-variant x 8,192-sample panel
 
 Prints one JSON line (rank 0, N = 1):
   value       evals/s = 5 * R(R+1)/2 / mean step seconds, R = 2 x variants
-  roofline    the count kernel: algorithmic lane-ops per launch / mean launch
-              time (HIP events on the scan stream) against the gfx950 VALU peak
+  roofline    the count kernel: algorithmic ops per launch / mean launch time
+              (HIP events on the scan stream) against the MI355X dense int8
+              MFMA peak (MFMA path) or the gfx950 VALU peak (popcount path)
   cpu_baseline  the oracle's OpenMP gold scan on a bounded leading subset of
               the same panel, on this host's cores
 """
@@ -25,6 +26,8 @@ sys.path.insert(0, ROOT)
 METRIC = "SNP-pair x Boolean-fn evals/s at 100k SNPs x 8k samples; p50/p99 scan latency"
 # gfx950: 256 CUs x 64 lanes x 2.4 GHz of 32-bit VALU lane operations
 VALU_PEAK_OPS = 256 * 64 * 2.4e9
+# MI355X dense int8 matrix peak
+INT8_MFMA_PEAK_OPS = 5.0e15
 
 
 def parse():
@@ -122,20 +125,26 @@ def main():
     total = time.perf_counter() - t0
     ms_per_step = 1000.0 * total / args.steps
 
-    # count kernel roofline: popcount lane-ops the kernel's algorithm needs
-    # = 9 products per variant pair x 2 ops (and, bcnt) x 32-bit words of data
+    # count kernel roofline.  Both paths form 9 plane products per variant pair
+    # over the n class samples.  Popcount: 2 VALU ops (and, bcnt) per 32-bit
+    # word of each product.  MFMA: 9 x n int8 MACs = 18 n ops per variant pair.
     var_pairs = n_var * (n_var + 1) // 2
-    words = (n0 + n1) / 32.0
-    ops_total = var_pairs * 9 * 2 * words
+    path = stats[-1]["count_path"]
     k_ms = statistics.mean(s["ms_count_kernels"] for s in stats)
     launches = statistics.mean(s["bands"] for s in stats)
+    if path == 2:
+        ops_total = var_pairs * 9 * 2.0 * (n0 + n1)
+        peak, bound, unit, kernel = INT8_MFMA_PEAK_OPS, "mfma", "TOP/s", "k_count_mfma"
+    else:
+        ops_total = var_pairs * 9 * 2 * ((n0 + n1) / 32.0)
+        peak, bound, unit, kernel = VALU_PEAK_OPS, "valu", "Tops/s", "k_count_popc"
     achieved = ops_total / (k_ms / 1000.0)
-    roof = {"bound": "valu", "kernel": "k_count_popc", "achieved": achieved / 1e12,
-            "peak": VALU_PEAK_OPS / 1e12, "unit": "Tops/s", "frac": achieved / VALU_PEAK_OPS,
-            "traffic": None,
+    roof = {"bound": bound, "kernel": kernel, "achieved": achieved / 1e12, "peak": peak / 1e12,
+            "unit": unit, "frac": achieved / peak, "traffic": None,
             "ops_per_launch": ops_total / launches, "launch_ms": k_ms / launches,
             "launches": launches,
-            "reference_unit_frac": (R * (R + 1) / 2) * (n / 4.0) / (k_ms / 1000.0) / VALU_PEAK_OPS}
+            "valu_popcount_roofline_frac": (R * (R + 1) / 2) * (n / 4.0) / (k_ms / 1000.0)
+            / VALU_PEAK_OPS}
     prof = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (roof["kernel"], args.workload))
     if os.path.exists(prof):
         roof["traffic"] = json.load(open(prof)).get("bytes_per_launch")
@@ -149,7 +158,8 @@ def main():
     line = {
         "metric": METRIC, "value": evals / statistics.mean(times), "unit": "evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i8" if path == 2 else "u32",
         "data": "synthetic",
         "config": {"workload": "%s: %d variants x %d samples (%d cases / %d controls), top-%d, %s"
                                % (args.workload, n_var, n, n1, n0, args.topk,

@@ -1,0 +1,269 @@
+/*
+ * gepi_count.h -- the two all-pairs count kernels (replace k1_count /
+ * k1_count_sm, src/kernels/kernel1.cu:60-284).  Both compute, for every
+ * variant pair (A >= B) of a tile and each class, the nine plane products
+ * c[cls][p][q] = popc(planeA[p] & planeB[q]) over the class's samples, then
+ * run the fused filter (gepi_device.h) on the 20 evaluations they determine.
+ *
+ * k_count_popc  VALU path: 32x32-variant tiles staged in LDS in 32-dword K
+ *               chunks; each thread owns 2x2 variant pairs and accumulates
+ *               and+popcount over 32-bit words (classes are word ranges).
+ * k_count_mfma  matrix-core path: the products are a Gram matrix of 0/1
+ *               plane rows, computed with v_mfma_i32_16x16x64_i8 on bytes
+ *               expanded from the packed bits in LDS.  One class is encoded
+ *               as 0x01 and the other as 0x80 (-128), so one int32
+ *               accumulator holds c_low + 16384 * c_high exactly (c_low <=
+ *               16383): both classes in one pass over K.  64x64-variant
+ *               workgroup tiles, 4 waves of 32x32, 64-sample K steps double
+ *               buffered in LDS.
+ */
+#ifndef GEPI_COUNT_H
+#define GEPI_COUNT_H
+
+#include "gepi_device.h"
+
+namespace gepi {
+
+/* ---------------------------------------------------------------------- */
+/* popcount path                                                           */
+
+constexpr int KC = 32;              /* dwords per K chunk staged in LDS          */
+constexpr int VSTRIDE = 3 * KC + 4; /* LDS dwords per variant (bank spread)      */
+constexpr int POPC_THREADS = 256;   /* 16 x 16                                   */
+
+template <int CLS>
+__device__ __forceinline__ void popc_step(const uint32_t *__restrict__ sA,
+                                          const uint32_t *__restrict__ sB, int ty, int tx, int kk,
+                                          uint32_t (&acc)[2][2][2][3][3]) {
+    uint4 a[2][3], b[2][3];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            a[i][p] = *reinterpret_cast<const uint4 *>(sA + (ty + 16 * i) * VSTRIDE + p * KC + kk);
+            b[i][p] = *reinterpret_cast<const uint4 *>(sB + (tx + 16 * i) * VSTRIDE + p * KC + kk);
+        }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    uint32_t s = acc[CLS][i][j][p][q];
+                    s += __popc(a[i][p].x & b[j][q].x);
+                    s += __popc(a[i][p].y & b[j][q].y);
+                    s += __popc(a[i][p].z & b[j][q].z);
+                    s += __popc(a[i][p].w & b[j][q].w);
+                    acc[CLS][i][j][p][q] = s;
+                }
+}
+
+template <int FFUN>
+__global__ void __launch_bounds__(POPC_THREADS) k_count_popc(CountArgs g) {
+    __shared__ __attribute__((aligned(16))) uint32_t sA[TV * VSTRIDE];
+    __shared__ __attribute__((aligned(16))) uint32_t sB[TV * VSTRIDE];
+    int32_t ta, tb;
+    tile_of(band_tile(g.tile_begin, g.n_tiles), ta, tb);
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    uint32_t acc[2][2][2][3][3];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) acc[c][i][j][p][q] = 0;
+
+    const uint32_t *baseA = g.planes + (int64_t)ta * TV * 3 * g.wc;
+    const uint32_t *baseB = g.planes + (int64_t)tb * TV * 3 * g.wc;
+    const int32_t nchunk = (g.wused + KC - 1) / KC;
+    for (int32_t c = 0; c < nchunk; c++) {
+        const int32_t c0 = c * KC;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            int idx = tid + r * POPC_THREADS; /* 0..767: (variant, plane, quad) */
+            int row = idx >> 3, quad = idx & 7;
+            int v = row / 3, p = row - 3 * v;
+            int64_t goff = ((int64_t)v * 3 + p) * g.wc + c0 + quad * 4;
+            int soff = v * VSTRIDE + p * KC + quad * 4;
+            *reinterpret_cast<uint4 *>(sA + soff) = *reinterpret_cast<const uint4 *>(baseA + goff);
+            *reinterpret_cast<uint4 *>(sB + soff) = *reinterpret_cast<const uint4 *>(baseB + goff);
+        }
+        __syncthreads();
+        int32_t end = g.wused - c0;
+        if (end > KC) end = KC;
+        int32_t split = g.w0 - c0;
+        if (split < 0) split = 0;
+        if (split > end) split = end;
+        for (int kk = 0; kk < split; kk += 4) popc_step<0>(sA, sB, ty, tx, kk, acc);
+        for (int kk = split; kk < end; kk += 4) popc_step<1>(sA, sB, ty, tx, kk, acc);
+    }
+
+    FilterConsts fc;
+    fc.thr_s = g.st->thr_score;
+    fc.kf = g.st->filt_k;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int32_t A = ta * TV + ty + 16 * i;
+            const int32_t B = tb * TV + tx + 16 * j;
+            if (A >= g.n_var || B > A) continue;
+            int32_t c[2][3][3];
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) c[k][p][q] = (int32_t)acc[k][i][j][p][q];
+            filter_pair<FFUN, false>(g, fc, A, B, c);
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* MFMA path                                                               */
+
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+
+constexpr int MF_THREADS = 256;  /* 4 waves, 2 x 2 of 32 x 32 variants          */
+constexpr int MF_ROWS = 2 * TM * 3; /* LDS rows per K step: A then B, 3 planes  */
+constexpr int MF_ROW = 80;       /* bytes per LDS row: 64 samples + 16 pad      */
+
+/* 64 packed samples -> 64 bytes (0x01 or 0x80 per set bit) at an LDS row.
+ * Nibble n spreads to bytes with (n * 0x204081) & 0x01010101.              */
+__device__ __forceinline__ void expand_row(uint8_t *__restrict__ row, uint64_t bits, bool high) {
+    const uint32_t lo = (uint32_t)bits, hi = (uint32_t)(bits >> 32);
+    uint32_t d[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        d[i] = (((lo >> (4 * i)) & 0xFu) * 0x204081u) & 0x01010101u;
+        d[8 + i] = (((hi >> (4 * i)) & 0xFu) * 0x204081u) & 0x01010101u;
+    }
+    if (high) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) d[i] <<= 7;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        *reinterpret_cast<uint4 *>(row + 16 * i) =
+            make_uint4(d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
+}
+
+template <int FFUN>
+__global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2][MF_ROWS * MF_ROW];
+    int32_t ta, tb;
+    tile_of(band_tile(g.tile_begin, g.n_tiles), ta, tb);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wa = wave >> 1, wb = wave & 1;
+    const int l16 = lane & 15, lg = lane >> 4;
+
+    v4i acc[2][2][3][3];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc[i][j][p][q] = v4i{0, 0, 0, 0};
+
+    /* staging: element e of a K step is row e of the LDS buffer; e < 192 is
+     * (A variant e/3, plane e%3), e >= 192 is B variant (e-192)/3.          */
+    const uint64_t *srcA = g.planes_k + (int64_t)ta * TM * 3;
+    const uint64_t *srcB = g.planes_k + (int64_t)tb * TM * 3;
+    const int64_t sstride = (int64_t)g.n_var_pad * 3;
+    const uint64_t *src0 = tid < 192 ? srcA + tid : srcB + (tid - 192);
+    const uint64_t *src1 = srcB + (tid + 64); /* element tid + 256, for tid < 128 */
+    uint64_t r0 = 0, r1 = 0;
+
+    r0 = src0[0];
+    if (tid < 128) r1 = src1[0];
+    {
+        const bool high = (0 < g.ks0) ? g.cls0_high : !g.cls0_high;
+        expand_row(&lds[0][tid * MF_ROW], r0, high);
+        if (tid < 128) expand_row(&lds[0][(tid + 256) * MF_ROW], r1, high);
+    }
+    __syncthreads();
+
+    for (int32_t s = 0; s < g.n_ks; s++) {
+        const int buf = s & 1;
+        const bool more = s + 1 < g.n_ks;
+        if (more) {
+            r0 = src0[(s + 1) * sstride];
+            if (tid < 128) r1 = src1[(s + 1) * sstride];
+        }
+        v4i a[2][3], b[2][3];
+        const uint8_t *L = lds[buf];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                a[i][p] = *reinterpret_cast<const v4i *>(
+                    L + ((32 * wa + 16 * i + l16) * 3 + p) * MF_ROW + 16 * lg);
+                b[i][p] = *reinterpret_cast<const v4i *>(
+                    L + (192 + (32 * wb + 16 * i + l16) * 3 + p) * MF_ROW + 16 * lg);
+            }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        acc[i][j][p][q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            a[i][p], b[j][q], acc[i][j][p][q], 0, 0, 0);
+        if (more) {
+            const bool high = (s + 1 < g.ks0) ? g.cls0_high : !g.cls0_high;
+            expand_row(&lds[buf ^ 1][tid * MF_ROW], r0, high);
+            if (tid < 128) expand_row(&lds[buf ^ 1][(tid + 256) * MF_ROW], r1, high);
+        }
+        __syncthreads();
+    }
+
+    /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
+     * variant 32wa + 16i + 4lg + r against B variant 32wb + 16j + l16.     */
+    FilterConsts fc;
+    fc.thr_s = g.st->thr_score;
+    fc.kf = g.st->filt_k;
+    const bool c0_high = g.cls0_high != 0; /* class 0 held in the high part */
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int32_t A = ta * TM + 32 * wa + 16 * i + 4 * lg + r;
+                const int32_t B = tb * TM + 32 * wb + 16 * j + l16;
+                if (A >= g.n_var || B > A) continue;
+                int32_t c[2][3][3];
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const int32_t S = acc[i][j][p][q][r];
+                        const int32_t hi = S >> 14, lo = S & 16383;
+                        c[0][p][q] = c0_high ? hi : lo;
+                        c[1][p][q] = c0_high ? lo : hi;
+                    }
+                filter_pair<FFUN, true>(g, fc, A, B, c);
+            }
+        }
+    }
+}
+
+} // namespace gepi
+
+#endif

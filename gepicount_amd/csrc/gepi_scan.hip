@@ -1,41 +1,28 @@
 /*
- * gepi_scan.hip -- device side of libgepicount: panel staging, the all-pairs
- * count kernel with fused scoring, and the on-device streaming top-k.
+ * gepi_scan.hip -- host orchestration and C ABI of libgepicount.
  *
- * Replaces src/kernels/kernel1.cu (launch_process_gpu_1 :295-505, k1_count
- * :182-284, k1_count_sm :60-180), src/kernels/entropyD.cu:31-50,
- * src/kernels/fisher_d.cu:290-311, src/kernels/misc1.cu (result buffers,
- * thrust sort, transferBack, mergeResults :35-295) and src/kernels/setup1.h.
+ * Replaces launch_process_gpu_1 and process_gpu_1 (src/kernels/kernel1.cu:295-516),
+ * the result-buffer management of src/kernels/misc1.cu:35-295 and the panel
+ * upload of src/kernels/setup1.h.  Kernels live in gepi_stage.h (panel
+ * staging), gepi_count.h (count + filter) and gepi_topk.h (exact scores and
+ * streaming top-k).
  *
  * Data layout in HBM
- *   panel   [n_var][3][W64] u64: per variant the allele-1 plane (row 2i of the
- *           reader, masked by valid), the allele-2 plane (row 2i+1) and the
- *           valid plane, in file sample order.  Resident across scans.
- *   classes [n_var_pad][3][Wc] u32: the same planes with samples reordered
- *           per scan: class-0 (control) samples first, then class-1 (case),
- *           each segment zero-padded to 128 bits.  Class masks disappear:
- *           a class is a word range.  n_var_pad is a multiple of the tile.
- *   cand    [cap] {score, idx, w, x, y, z}: candidates for the top-k.
+ *   panel    [n_var][3][W64] u64: per variant the allele-1 plane (reader row
+ *            2i masked by valid), allele-2 plane (row 2i+1) and valid plane,
+ *            file sample order.  Resident across scans.
+ *   classes  [n_var_pad][3][wc] u32 per scan: samples reordered so that class
+ *            0 (controls) then class 1 (cases) are contiguous, each segment
+ *            zero-padded to 128 bits; class masks become word ranges.
+ *   classes_k [n_ks][n_var_pad][3] u64 (MFMA path): the same bits K-major.
+ *   pend     [cap] 16-byte filter survivors; cand [2][cap] 32-byte candidates.
  *
- * Counting.  For variant pair (A, B) and class c the kernel forms the nine
- * popcounts prod[p][q] = popc(planeA[p] & planeB[q]) over the class's words
- * (p, q in {allele1, allele2, valid}).  Every row pair of the two variants
- * and every Boolean function follows exactly, because allele planes are
- * subsets of their valid plane:
- *   N11 = prod[ia][ib], NA = prod[ia][V], NB = prod[V][ib], V = prod[V][V]
- *   AND = N11, NAND = NB-N11, ANDN = NA-N11, NANDN = V-NA-NB+N11,
- *   XOR = NA+NB-2*N11,    y = V(class 0), z = V(class 1).
- *
- * Scoring.  Each evaluation passes a conservative single-precision filter
- * against the current admission threshold; survivors get the exact score
- * (gepi_score.h) and are appended when their key (score, sA, sB, fun) is <=
- * the threshold key.  Entropy filter: -I <= t requires I >= -t, and
- * I_bits <= log2(1 + chi2/m) gives (wz - xy)^2 >= (2^I - 1) y z xx (m - xx).
- * Fisher filter: p >= P(observed table).
- *
- * Top-k.  A band of tiles is counted, then a radix select over the 128-bit
- * keys finds the k-th smallest key, the buffer is compacted to those k and
- * the key becomes the new threshold, so only k records ever leave the GPU.
+ * A scan walks the lower triangle of variant tiles in bands: count kernel
+ * (filter survivors -> pend), k_exact (exact score, admit keys <= threshold
+ * -> cand), then, when more than k candidates exist, a radix select of the
+ * k-th key, compaction to exactly k records and a new threshold.  The host
+ * synchronises once per band to read the state and size the next band; only
+ * the final k records are copied to the host.
  */
 #include <hip/hip_runtime.h>
 
@@ -50,10 +37,15 @@
 #include <vector>
 
 #include "../../include/gepicount.h"
+#include "gepi_count.h"
 #include "gepi_internal.h"
 #include "gepi_score.h"
+#include "gepi_stage.h"
+#include "gepi_topk.h"
 
-#define GEPI_VERSION "gepicount-mi355x 0.1.0"
+#define GEPI_VERSION "gepicount-mi355x 0.2.0"
+
+using namespace gepi;
 
 /* ------------------------------------------------------------------------ */
 /* errors                                                                    */
@@ -80,488 +72,12 @@ extern "C" const char *gepi_version(void) { return GEPI_VERSION; }
     } while (0)
 
 /* ------------------------------------------------------------------------ */
-/* constants and device structs                                              */
-namespace {
-
-constexpr int TV = 32;                 /* variants per tile side                 */
-constexpr int KC = 32;                 /* dwords per K chunk staged in LDS       */
-constexpr int VSTRIDE = 3 * KC + 4;    /* LDS dwords per variant (bank spread)   */
-constexpr int COUNT_THREADS = 256;     /* 16 x 16                                */
-constexpr int SEL_SMALL = 1024;        /* finish the select in LDS below this    */
-
-struct Cand {
-    double score;
-    uint64_t idx;
-    uint32_t w, x, y, z;
-};
-static_assert(sizeof(Cand) == 32, "candidate record is 32 bytes");
-
-struct ScanState {
-    double thr_score;            /* admission threshold key: (thr_score, thr_idx) */
-    uint64_t thr_idx;
-    float filt_k;                /* entropy filter coefficient, 0 = pass all      */
-    uint32_t overflow;
-    unsigned long long count;    /* records appended to the active buffer         */
-    unsigned long long count2;   /* records written by the compaction             */
-    unsigned long long exact;    /* exact evaluations performed                   */
-    /* radix select */
-    uint64_t key_hi, key_lo;
-    uint32_t bits_done, rank, matched, done, gather_n, pad0;
-    uint32_t hist[256];
-};
-
-struct CountArgs {
-    const uint32_t *planes;      /* [n_var_pad][3][wc]                            */
-    int64_t tile_begin;          /* first linear tile of the band                 */
-    int32_t n_tiles;             /* tiles in the band (= grid size)               */
-    int32_t n_var;
-    int32_t wc;                  /* dwords per plane row (multiple of KC)         */
-    int32_t w0;                  /* class-0 dwords                                */
-    int32_t wused;               /* dwords holding data (multiple of 4)           */
-    int32_t ffun;
-    ScanState *st;
-    Cand *cand;
-    unsigned long long cap;
-    const double *lhi, *llo;     /* t*log2 t, double-double                       */
-    const double *lfact;         /* ln t!                                         */
-};
-
-__host__ __device__ inline void tile_of(int64_t t, int32_t &ta, int32_t &tb) {
-    /* t = ta*(ta+1)/2 + tb, 0 <= tb <= ta */
-    double r = sqrt(8.0 * (double)t + 1.0);
-    int64_t a = (int64_t)((r - 1.0) * 0.5);
-    while (a * (a + 1) / 2 > t) a--;
-    while ((a + 1) * (a + 2) / 2 <= t) a++;
-    ta = (int32_t)a;
-    tb = (int32_t)(t - a * (a + 1) / 2);
-}
-
-/* ------------------------------------------------------------------------ */
-/* panel staging kernels                                                     */
-
-/* .bed payload -> [n_var][3][W64] planes (BinPlinkReader.h:171-201). */
-__global__ void k_decode_bed(const uint8_t *__restrict__ bed, int64_t n_var, int32_t n_samples,
-                             int32_t w64, int64_t bps, uint64_t *__restrict__ nat) {
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n_var * w64) return;
-    int64_t v = gid / w64;
-    int32_t wd = (int32_t)(gid % w64);
-    const uint8_t *src = bed + v * bps;
-    uint64_t a1 = 0, a2 = 0, vv = 0;
-    int32_t s0 = wd * 64;
-    int32_t s1 = s0 + 64 < n_samples ? s0 + 64 : n_samples;
-    for (int32_t s = s0; s < s1; s += 4) {
-        uint32_t byte = src[s >> 2];
-        for (int32_t q = 0; q < 4 && s + q < s1; q++) {
-            uint32_t code = (byte >> (2 * q)) & 3u;
-            uint64_t bit = 1ull << ((s + q) & 63);
-            if (code == 1u) continue;
-            vv |= bit;
-            if (code != 3u) a1 |= bit;
-            if (code != 0u) a2 |= bit;
-        }
-    }
-    uint64_t *dst = nat + (v * 3) * w64 + wd;
-    dst[0] = a1;
-    dst[w64] = a2;
-    dst[2 * w64] = vv;
-}
-
-/* Class layout: out[v][p][d] bit b = panel bit perm[32*d + b] (or 0). */
-__global__ void __launch_bounds__(256) k_gather_class(const uint64_t *__restrict__ nat,
-                                                       int32_t n_var, int32_t w64,
-                                                       const int32_t *__restrict__ perm,
-                                                       int32_t wc, int32_t wused,
-                                                       uint32_t *__restrict__ out) {
-    extern __shared__ uint64_t srow[];
-    const int32_t v = blockIdx.x;
-    const bool real = v < n_var;
-    for (int32_t i = threadIdx.x; i < 3 * w64; i += blockDim.x)
-        srow[i] = real ? nat[(int64_t)v * 3 * w64 + i] : 0ull;
-    __syncthreads();
-    for (int32_t i = threadIdx.x; i < 3 * wc; i += blockDim.x) {
-        int32_t p = i / wc, d = i - p * wc;
-        uint32_t word = 0;
-        if (real && d < wused) {
-            const int32_t *pp = perm + 32 * d;
-            const uint64_t *row = srow + p * w64;
-#pragma unroll 8
-            for (int32_t b = 0; b < 32; b++) {
-                int32_t s = pp[b];
-                if (s >= 0) word |= (uint32_t)((row[s >> 6] >> (s & 63)) & 1ull) << b;
-            }
-        }
-        out[((int64_t)v * 3 + p) * wc + d] = word;
-    }
-}
-
-/* ------------------------------------------------------------------------ */
-/* count kernel                                                              */
-
-template <int CLS>
-__device__ __forceinline__ void popc_step(const uint32_t *__restrict__ sA,
-                                          const uint32_t *__restrict__ sB, int ty, int tx, int kk,
-                                          uint32_t (&acc)[2][2][2][3][3]) {
-    uint4 a[2][3], b[2][3];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int p = 0; p < 3; p++) {
-            a[i][p] = *reinterpret_cast<const uint4 *>(sA + (ty + 16 * i) * VSTRIDE + p * KC + kk);
-            b[i][p] = *reinterpret_cast<const uint4 *>(sB + (tx + 16 * i) * VSTRIDE + p * KC + kk);
-        }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    uint32_t s = acc[CLS][i][j][p][q];
-                    s += __popc(a[i][p].x & b[j][q].x);
-                    s += __popc(a[i][p].y & b[j][q].y);
-                    s += __popc(a[i][p].z & b[j][q].z);
-                    s += __popc(a[i][p].w & b[j][q].w);
-                    acc[CLS][i][j][p][q] = s;
-                }
-}
-
-__device__ __forceinline__ void admit(const CountArgs &g, double s, uint64_t idx, double thr_s,
-                                      uint64_t thr_i, int32_t w, int32_t x, int32_t y, int32_t z) {
-    if (s < thr_s || (s == thr_s && idx <= thr_i)) {
-        unsigned long long pos = atomicAdd(&g.st->count, 1ull);
-        if (pos < g.cap) {
-            Cand c;
-            c.score = s;
-            c.idx = idx;
-            c.w = (uint32_t)w;
-            c.x = (uint32_t)x;
-            c.y = (uint32_t)y;
-            c.z = (uint32_t)z;
-            g.cand[pos] = c;
-        } else {
-            atomicOr(&g.st->overflow, 1u);
-        }
-    }
-}
-
-template <int FFUN>
-__global__ void __launch_bounds__(COUNT_THREADS) k_count_popc(CountArgs g) {
-    __shared__ __attribute__((aligned(16))) uint32_t sA[TV * VSTRIDE];
-    __shared__ __attribute__((aligned(16))) uint32_t sB[TV * VSTRIDE];
-
-    /* XCD-aware band mapping: consecutive workgroups land on different XCDs
-     * (round robin); give each XCD a contiguous run of tiles so that tiles
-     * sharing a row panel share that XCD's L2.                              */
-    const int32_t nb = g.n_tiles;
-    const int32_t gidx = blockIdx.x;
-    const int32_t xcd = gidx & 7, slot = gidx >> 3;
-    const int32_t per = nb >> 3, rem = nb & 7;
-    const int64_t t = g.tile_begin + (int64_t)xcd * per + (xcd < rem ? xcd : rem) + slot;
-    int32_t ta, tb;
-    tile_of(t, ta, tb);
-
-    const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    uint32_t acc[2][2][2][3][3];
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-#pragma unroll
-                for (int p = 0; p < 3; p++)
-#pragma unroll
-                    for (int q = 0; q < 3; q++) acc[c][i][j][p][q] = 0;
-
-    const uint32_t *baseA = g.planes + (int64_t)ta * TV * 3 * g.wc;
-    const uint32_t *baseB = g.planes + (int64_t)tb * TV * 3 * g.wc;
-    const int32_t nchunk = (g.wused + KC - 1) / KC;
-    for (int32_t c = 0; c < nchunk; c++) {
-        const int32_t c0 = c * KC;
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            int idx = tid + r * COUNT_THREADS; /* 0..767: (variant, plane, quad) */
-            int row = idx >> 3, quad = idx & 7;
-            int v = row / 3, p = row - 3 * v;
-            int64_t goff = ((int64_t)v * 3 + p) * g.wc + c0 + quad * 4;
-            int soff = v * VSTRIDE + p * KC + quad * 4;
-            *reinterpret_cast<uint4 *>(sA + soff) = *reinterpret_cast<const uint4 *>(baseA + goff);
-            *reinterpret_cast<uint4 *>(sB + soff) = *reinterpret_cast<const uint4 *>(baseB + goff);
-        }
-        __syncthreads();
-        int32_t end = g.wused - c0;
-        if (end > KC) end = KC;
-        int32_t split = g.w0 - c0;
-        if (split < 0) split = 0;
-        if (split > end) split = end;
-        for (int kk = 0; kk < split; kk += 4) popc_step<0>(sA, sB, ty, tx, kk, acc);
-        for (int kk = split; kk < end; kk += 4) popc_step<1>(sA, sB, ty, tx, kk, acc);
-    }
-
-    /* epilogue: scoring of 4 variant pairs x (4 or 3 row pairs) x 5 functions */
-    const double thr_s = g.st->thr_score;
-    const uint64_t thr_i = g.st->thr_idx;
-    const float kf = g.st->filt_k;
-    const float lo_margin = 1.0f - 1.0f / 65536.0f;
-    uint32_t n_exact = 0;
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int32_t A = ta * TV + ty + 16 * i;
-            const int32_t B = tb * TV + tx + 16 * j;
-            if (A >= g.n_var || B > A) continue;
-            const int32_t y = (int32_t)acc[0][i][j][2][2];
-            const int32_t z = (int32_t)acc[1][i][j][2][2];
-            const int32_t m = y + z;
-            const float kyz = kf * (float)(y * z);
-#pragma unroll
-            for (int ia = 0; ia < 2; ia++) {
-#pragma unroll
-                for (int ib = 0; ib < 2; ib++) {
-                    if (A == B && ia < ib) continue;
-                    const int32_t n0 = (int32_t)acc[0][i][j][ia][ib];
-                    const int32_t a0 = (int32_t)acc[0][i][j][ia][2];
-                    const int32_t b0 = (int32_t)acc[0][i][j][2][ib];
-                    const int32_t n1 = (int32_t)acc[1][i][j][ia][ib];
-                    const int32_t a1 = (int32_t)acc[1][i][j][ia][2];
-                    const int32_t b1 = (int32_t)acc[1][i][j][2][ib];
-                    const int64_t sA_ = 2 * (int64_t)A + ia;
-                    const int64_t sB_ = 2 * (int64_t)B + ib;
-                    int32_t ws[5] = {n0, b0 - n0, a0 - n0, y - a0 - b0 + n0, a0 + b0 - 2 * n0};
-                    int32_t xs[5] = {n1, b1 - n1, a1 - n1, z - a1 - b1 + n1, a1 + b1 - 2 * n1};
-#pragma unroll
-                    for (int f = 0; f < 5; f++) {
-                        const int32_t w = ws[f], x = xs[f];
-                        bool pass;
-                        if (FFUN == GEPI_FFUNC_ENTROPY) {
-                            const int32_t xx = w + x;
-                            const int32_t q = xx * (m - xx);
-                            const int32_t d = w * z - x * y;
-                            const float fd = (float)d;
-                            /* d == 0 is an independent table (I = 0): it only
-                             * qualifies while the threshold admits everything */
-                            pass = (kf == 0.0f) || (d != 0 && fd * fd >= kyz * (float)q * lo_margin);
-                        } else {
-                            pass = true;
-                            if (thr_s < 0.0) {
-                                const double *lf = g.lfact;
-                                double lp = ((lf[y] + lf[z]) + (lf[w + x] + lf[m - w - x])) -
-                                            (((lf[w] + lf[y - w]) + (lf[x] + lf[z - x])) + lf[m]);
-                                pass = lp * 0.4342944819032518 - 1e-6 <= thr_s;
-                            }
-                        }
-                        if (pass) {
-                            n_exact++;
-                            const double s =
-                                (FFUN == GEPI_FFUNC_ENTROPY)
-                                    ? gepi::entropy_exact(w, x, y, z, g.lhi, g.llo)
-                                    : gepi::fisher_exact(w, x, y, z);
-                            admit(g, s, gepi::pack_idx(sA_, sB_, f), thr_s, thr_i, w, x, y, z);
-                        }
-                    }
-                }
-            }
-        }
-    }
-    if (n_exact) atomicAdd(&g.st->exact, (unsigned long long)n_exact);
-}
-
-/* ------------------------------------------------------------------------ */
-/* top-k maintenance: radix select over 128-bit keys (score bits, idx)       */
-
-struct SelPrefix {
-    uint64_t key_hi, key_lo;
-    uint32_t bits_done;
-};
-
-__device__ __forceinline__ SelPrefix load_prefix(const ScanState *st) {
-    SelPrefix s;
-    s.key_hi = st->key_hi;
-    s.key_lo = st->key_lo;
-    s.bits_done = st->bits_done;
-    return s;
-}
-
-__device__ __forceinline__ bool key_match(uint64_t hi, uint64_t lo, const SelPrefix &s) {
-    uint32_t b = s.bits_done;
-    if (b == 0) return true;
-    if (b < 64) return (hi >> (64 - b)) == (s.key_hi >> (64 - b));
-    if (hi != s.key_hi) return false;
-    if (b == 64) return true;
-    return (lo >> (128 - b)) == (s.key_lo >> (128 - b));
-}
-
-__device__ __forceinline__ uint32_t key_digit(uint64_t hi, uint64_t lo, uint32_t b) {
-    return b < 64 ? (uint32_t)((hi >> (56 - b)) & 0xFF) : (uint32_t)((lo >> (120 - b)) & 0xFF);
-}
-
-__global__ void __launch_bounds__(256) k_sel_hist(const Cand *__restrict__ cand, ScanState *st) {
-    if (st->done) return;
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const SelPrefix s = load_prefix(st);
-    const unsigned long long n = st->count;
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (unsigned long long)gridDim.x * blockDim.x) {
-        uint64_t hi = gepi::order_bits(cand[i].score), lo = cand[i].idx;
-        if (key_match(hi, lo, s)) atomicAdd(&h[key_digit(hi, lo, s.bits_done)], 1u);
-    }
-    __syncthreads();
-    if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], h[threadIdx.x]);
-}
-
-__global__ void k_sel_pick(ScanState *st) {
-    if (threadIdx.x != 0 || st->done) return;
-    uint32_t cum = 0, d = 0;
-    for (d = 0; d < 256; d++) {
-        uint32_t c = st->hist[d];
-        if (cum + c >= st->rank) break;
-        cum += c;
-    }
-    if (d == 256) { /* rank beyond the matching records: cannot happen for rank <= count */
-        st->done = 3;
-        return;
-    }
-    uint32_t b = st->bits_done;
-    if (b < 64)
-        st->key_hi |= (uint64_t)d << (56 - b);
-    else
-        st->key_lo |= (uint64_t)d << (120 - b);
-    st->rank -= cum;
-    st->matched = st->hist[d];
-    st->bits_done = b + 8;
-    for (int i = 0; i < 256; i++) st->hist[i] = 0;
-    if (st->bits_done == 128)
-        st->done = 1;
-    else if (st->matched <= SEL_SMALL)
-        st->done = 2;
-}
-
-__global__ void __launch_bounds__(256) k_sel_gather(const Cand *__restrict__ cand, ScanState *st,
-                                                     uint64_t *small) {
-    if (st->done != 2) return;
-    const SelPrefix s = load_prefix(st);
-    const unsigned long long n = st->count;
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-         i < n; i += (unsigned long long)gridDim.x * blockDim.x) {
-        uint64_t hi = gepi::order_bits(cand[i].score), lo = cand[i].idx;
-        if (key_match(hi, lo, s)) {
-            uint32_t p = atomicAdd(&st->gather_n, 1u);
-            if (p < SEL_SMALL) {
-                small[2 * p] = hi;
-                small[2 * p + 1] = lo;
-            }
-        }
-    }
-}
-
-__global__ void __launch_bounds__(1024) k_sel_small(ScanState *st, const uint64_t *small) {
-    if (st->done != 2) return;
-    __shared__ uint64_t kh[SEL_SMALL], kl[SEL_SMALL];
-    const uint32_t n = st->gather_n < SEL_SMALL ? st->gather_n : SEL_SMALL;
-    const int t = threadIdx.x;
-    kh[t] = t < (int)n ? small[2 * t] : ~0ull;
-    kl[t] = t < (int)n ? small[2 * t + 1] : ~0ull;
-    __syncthreads();
-    for (int size = 2; size <= SEL_SMALL; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            int partner = t ^ stride;
-            if (partner > t) {
-                bool up = (t & size) == 0;
-                bool gt = kh[t] > kh[partner] || (kh[t] == kh[partner] && kl[t] > kl[partner]);
-                if (gt == up) {
-                    uint64_t a = kh[t], b = kl[t];
-                    kh[t] = kh[partner];
-                    kl[t] = kl[partner];
-                    kh[partner] = a;
-                    kl[partner] = b;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (t == 0) {
-        uint32_t r = st->rank;
-        st->key_hi = kh[r - 1];
-        st->key_lo = kl[r - 1];
-        st->bits_done = 128;
-        st->done = 1;
-    }
-}
-
-/* Keeps records with key <= selected key, optionally dropping a tile range. */
-__global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, ScanState *st,
-                                                  Cand *__restrict__ dst, int64_t drop_t0,
-                                                  int64_t drop_t1) {
-    const uint64_t khi = st->key_hi, klo = st->key_lo;
-    const unsigned long long n = st->count;
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (unsigned long long)gridDim.x * blockDim.x) {
-        Cand c = src[i];
-        uint64_t hi = gepi::order_bits(c.score);
-        bool keep = hi < khi || (hi == khi && c.idx <= klo);
-        if (keep && drop_t1 > drop_t0) {
-            int64_t a = (int64_t)(c.idx >> 32) >> 1;
-            int64_t b = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull) >> 1;
-            int64_t ta = a / TV, tb = b / TV;
-            int64_t t = ta * (ta + 1) / 2 + tb;
-            keep = t < drop_t0 || t >= drop_t1;
-        }
-        if (keep) {
-            unsigned long long p = atomicAdd(&st->count2, 1ull);
-            dst[p] = c;
-        }
-    }
-}
-
-/* After a compaction: adopt the compacted buffer and the new threshold. */
-__global__ void k_finalize(ScanState *st, int32_t ffun) {
-    if (threadIdx.x != 0) return;
-    st->count = st->count2;
-    st->count2 = 0;
-    double thr = gepi::from_order_bits(st->key_hi);
-    st->thr_score = thr;
-    st->thr_idx = st->key_lo;
-    float kf = 0.0f;
-    if (ffun == GEPI_FFUNC_ENTROPY) {
-        double it = -thr - 1e-12;
-        if (it > 0.0) {
-            double k = expm1(it * 0.6931471805599453) * (1.0 - 1e-6);
-            kf = (float)k;
-            if ((double)kf > k) kf = nextafterf(kf, 0.0f);
-        }
-    }
-    st->filt_k = kf;
-}
-
-/* Resets the select state for a new select over `count` records. */
-__global__ void k_sel_reset(ScanState *st, uint32_t rank) {
-    if (threadIdx.x != 0) return;
-    st->key_hi = 0;
-    st->key_lo = 0;
-    st->bits_done = 0;
-    st->rank = rank;
-    st->matched = 0;
-    st->done = 0;
-    st->gather_n = 0;
-    for (int i = 0; i < 256; i++) st->hist[i] = 0;
-}
-
-/* ------------------------------------------------------------------------ */
-/* host side                                                                 */
+/* panel                                                                     */
 
 struct DeviceTables {
     int32_t tmax = -1;
     double *lhi = nullptr, *llo = nullptr, *lfact = nullptr;
 };
-
-} // namespace
 
 struct gepi_panel {
     int device = 0;
@@ -570,13 +86,16 @@ struct gepi_panel {
     int32_t w64 = 0;
     uint64_t *d_nat = nullptr;
     hipStream_t stream = nullptr;
-    /* per-scan workspace, grown on demand */
+    /* per-scan workspace, grown on demand and reused */
     uint32_t *d_cls = nullptr;
     size_t cls_bytes = 0;
+    uint64_t *d_clsk = nullptr;
+    size_t clsk_bytes = 0;
     int32_t *d_perm = nullptr;
     size_t perm_bytes = 0;
     Cand *d_cand[2] = {nullptr, nullptr};
-    unsigned long long cand_cap = 0;
+    Pend *d_pend = nullptr;
+    unsigned long long buf_cap = 0;
     ScanState *d_state = nullptr;
     uint64_t *d_small = nullptr;
     DeviceTables tabs;
@@ -654,9 +173,11 @@ void panel_free(gepi_panel *p) {
     if (p->stream) hipStreamSynchronize(p->stream);
     hipFree(p->d_nat);
     hipFree(p->d_cls);
+    hipFree(p->d_clsk);
     hipFree(p->d_perm);
     hipFree(p->d_cand[0]);
     hipFree(p->d_cand[1]);
+    hipFree(p->d_pend);
     hipFree(p->d_state);
     hipFree(p->d_small);
     hipFree(p->tabs.lhi);
@@ -690,7 +211,7 @@ int validate_params(const gepi_params *par) {
     return GEPI_OK;
 }
 
-/* Select the rank-th smallest key (1-based) among st->count records of src. */
+/* Select the rank-th smallest key (1-based) among the count records of src. */
 void launch_select(gepi_panel *p, const Cand *src, uint32_t rank, unsigned long long count) {
     k_sel_reset<<<1, 1, 0, p->stream>>>(p->d_state, rank);
     unsigned long long blocks = (count + 255) / 256;
@@ -704,19 +225,14 @@ void launch_select(gepi_panel *p, const Cand *src, uint32_t rank, unsigned long 
     k_sel_small<<<1, SEL_SMALL, 0, p->stream>>>(p->d_state, p->d_small);
 }
 
-struct HostState {
-    unsigned long long count, exact;
-    uint32_t overflow;
-};
-
-int read_state(gepi_panel *p, HostState *hs) {
-    ScanState s;
-    HIP_TRY(hipMemcpyAsync(&s, p->d_state, sizeof(ScanState), hipMemcpyDeviceToHost, p->stream));
+int read_state(gepi_panel *p, ScanState *s) {
+    HIP_TRY(hipMemcpyAsync(s, p->d_state, sizeof(ScanState), hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
-    hs->count = s.count;
-    hs->exact = s.exact;
-    hs->overflow = s.overflow;
     return GEPI_OK;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
 
 int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case,
@@ -728,8 +244,6 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     if (!n_tops) return gepi_fail(GEPI_ERR_ARG, "null n_tops");
     *n_tops = 0;
     if (par->topmost > 0 && !tops) return gepi_fail(GEPI_ERR_ARG, "null tops");
-    if (par->count_path == 2)
-        return gepi_fail(GEPI_ERR_ARG, "MFMA count path not available in this build");
     if ((rc = set_device(p->device)) != GEPI_OK) return rc;
 
     gepi_scan_stats st_local;
@@ -738,9 +252,8 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const int64_t R = 2 * n_var;
     const int64_t total_evals = 5 * (R * (R + 1) / 2);
     st_local.evals = total_evals;
-    st_local.count_path = 1;
 
-    /* class layout (sample permutation) */
+    /* class layout (sample permutation); a sample in both masks counts in both */
     std::vector<int32_t> cls0, cls1;
     for (int32_t s = 0; s < p->n_samples; s++) {
         uint64_t bit = 1ull << (s & 63);
@@ -753,8 +266,29 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     if ((int64_t)n0 + n1 > GEPI_MAX_SAMPLES)
         return gepi_fail(GEPI_ERR_LIMIT, "%d class samples exceed the limit of %d", n0 + n1,
                          GEPI_MAX_SAMPLES);
+
+    /* count path: MFMA needs one class of at most MFMA_LOW_MAX samples in the
+     * low part of the accumulator; the larger class goes there when it fits. */
+    int cls0_high = -1;
+    {
+        const int big0 = n0 >= n1;
+        const int32_t nbig = big0 ? n0 : n1, nsmall = big0 ? n1 : n0;
+        if (nbig <= MFMA_LOW_MAX)
+            cls0_high = big0 ? 0 : 1;
+        else if (nsmall <= MFMA_LOW_MAX)
+            cls0_high = big0 ? 1 : 0;
+    }
+    int path = par->count_path;
+    if (path == 0) path = cls0_high >= 0 ? 2 : 1;
+    if (path == 2 && cls0_high < 0)
+        return gepi_fail(GEPI_ERR_LIMIT,
+                         "MFMA count path needs a class with at most %d samples (have %d and %d)",
+                         MFMA_LOW_MAX, n0, n1);
+    st_local.count_path = path;
+
     const int64_t k = par->topmost < total_evals ? par->topmost : total_evals;
     if (k == 0) {
+        st_local.ms_scan = ms_since(t_start);
         if (stats) *stats = st_local;
         return GEPI_OK;
     }
@@ -762,31 +296,40 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const int32_t w0 = ((n0 + 127) / 128) * 4; /* dwords, 128-bit padded */
     const int32_t w1 = ((n1 + 127) / 128) * 4;
     const int32_t wused = w0 + w1;
-    const int32_t wc = ((wused + KC - 1) / KC) * KC > 0 ? ((wused + KC - 1) / KC) * KC : KC;
+    const int32_t wc = std::max(KC, ((wused + KC - 1) / KC) * KC);
     std::vector<int32_t> perm((size_t)wc * 32, -1);
     for (int32_t i = 0; i < n0; i++) perm[i] = cls0[i];
     for (int32_t i = 0; i < n1; i++) perm[(size_t)w0 * 32 + i] = cls1[i];
-    const int64_t nt_side = (n_var + TV - 1) / TV;
-    const int64_t n_var_pad = nt_side * TV;
+    const int32_t tv = path == 2 ? TM : TV;
+    const int64_t n_var_pad = ((n_var + TM - 1) / TM) * TM;
+    const int64_t nt_side = n_var_pad / tv;
     const int64_t n_tiles = nt_side * (nt_side + 1) / 2;
+    const int32_t n_ks = wused / 2;
 
     if ((rc = ensure_tables(p, n0 + n1 + 1)) != GEPI_OK) return rc;
     if ((rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK) return rc;
+    if (path == 2 &&
+        (rc = grow(&p->d_clsk, &p->clsk_bytes,
+                   (size_t)std::max(n_ks, 1) * n_var_pad * 3 * 8)) != GEPI_OK)
+        return rc;
     if ((rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK) return rc;
     unsigned long long cap = (unsigned long long)std::max<int64_t>(4 * k, 1 << 23);
     if (const char *env = getenv("GEPI_CAND_CAP")) {
         /* capacity override for exercising the overflow recovery in tests */
-        long long want = atoll(env);
-        cap = (unsigned long long)std::max<long long>(want, 2 * k + 1024);
+        cap = (unsigned long long)std::max<long long>(atoll(env), 2 * k + 1024);
+        cap = std::max<unsigned long long>(cap, (unsigned long long)TM * TM * 20 + 1024);
     }
-    if (cap > p->cand_cap) {
+    if (cap > p->buf_cap) {
         hipFree(p->d_cand[0]);
         hipFree(p->d_cand[1]);
+        hipFree(p->d_pend);
         p->d_cand[0] = p->d_cand[1] = nullptr;
-        p->cand_cap = 0;
+        p->d_pend = nullptr;
+        p->buf_cap = 0;
         HIP_TRY(hipMalloc(&p->d_cand[0], sizeof(Cand) * cap));
         HIP_TRY(hipMalloc(&p->d_cand[1], sizeof(Cand) * cap));
-        p->cand_cap = cap;
+        HIP_TRY(hipMalloc(&p->d_pend, sizeof(Pend) * cap));
+        p->buf_cap = cap;
     }
 
     HIP_TRY(hipMemcpyAsync(p->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice,
@@ -797,6 +340,12 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         k_gather_class<<<(unsigned)n_var_pad, 256, lds, p->stream>>>(
             p->d_nat, (int32_t)n_var, p->w64, p->d_perm, wc, wused, p->d_cls);
         HIP_TRY(hipGetLastError());
+        if (path == 2 && n_ks > 0) {
+            dim3 grid((unsigned)(n_var_pad / 32), (unsigned)((n_ks + 31) / 32));
+            k_transpose_k<<<grid, 256, 0, p->stream>>>(p->d_cls, (int32_t)n_var_pad, wc, n_ks,
+                                                       p->d_clsk);
+            HIP_TRY(hipGetLastError());
+        }
     }
     HIP_TRY(hipEventRecord(p->ev1, p->stream));
 
@@ -804,7 +353,6 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     memset(&init, 0, sizeof init);
     init.thr_score = INFINITY;
     init.thr_idx = ~0ull;
-    init.filt_k = 0.0f;
     HIP_TRY(hipMemcpyAsync(p->d_state, &init, sizeof init, hipMemcpyHostToDevice, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
     {
@@ -815,73 +363,106 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
 
     int cur = 0; /* active candidate buffer */
     CountArgs ca;
+    memset(&ca, 0, sizeof ca);
     ca.planes = p->d_cls;
+    ca.planes_k = p->d_clsk;
     ca.n_var = (int32_t)n_var;
+    ca.n_var_pad = (int32_t)n_var_pad;
     ca.wc = wc;
     ca.w0 = w0;
     ca.wused = wused;
-    ca.ffun = par->ffun;
+    ca.n_ks = n_ks;
+    ca.ks0 = w0 / 2;
+    ca.cls0_high = cls0_high > 0 ? 1 : 0;
     ca.st = p->d_state;
-    ca.cap = cap;
-    ca.lhi = p->tabs.lhi;
-    ca.llo = p->tabs.llo;
+    ca.pend = p->d_pend;
+    ca.pcap = cap;
     ca.lfact = p->tabs.lfact;
+    ExactArgs ea;
+    ea.st = p->d_state;
+    ea.pend = p->d_pend;
+    ea.pcap = cap;
+    ea.cap = cap;
+    ea.lhi = p->tabs.lhi;
+    ea.llo = p->tabs.llo;
 
-    const int64_t evals_per_tile = (int64_t)TV * TV * 20;
-    int64_t max_band = 65536;
+    const int64_t evals_per_tile = (int64_t)tv * tv * 20;
+    int64_t max_band = std::max<int64_t>(64, (int64_t)(2.0e9 / (double)evals_per_tile));
     if (par->max_grid_area > 0)
         max_band = std::max<int64_t>(1, ((int64_t)par->max_grid_area << 20) / evals_per_tile);
     int64_t band = std::max<int64_t>(1, std::min<int64_t>(max_band, (2 * k) / evals_per_tile + 1));
-    double ms_count = 0.0, ms_sel = 0.0;
+    double ms_count = 0.0, ms_top = 0.0;
     int64_t t0 = 0;
     unsigned long long count = 0, exact_total = 0;
     int bands = 0;
+    const bool entropy = par->ffun == GEPI_FFUNC_ENTROPY;
     while (t0 < n_tiles) {
-        int64_t t1 = std::min<int64_t>(n_tiles, t0 + band);
+        const int64_t t1 = std::min<int64_t>(n_tiles, t0 + band);
         ca.tile_begin = t0;
         ca.n_tiles = (int32_t)(t1 - t0);
-        ca.cand = p->d_cand[cur];
+        ea.cand = p->d_cand[cur];
+        k_reset_pending<<<1, 1, 0, p->stream>>>(p->d_state);
         HIP_TRY(hipEventRecord(p->ev0, p->stream));
-        if (par->ffun == GEPI_FFUNC_ENTROPY)
-            k_count_popc<GEPI_FFUNC_ENTROPY><<<(unsigned)(t1 - t0), COUNT_THREADS, 0, p->stream>>>(ca);
-        else
-            k_count_popc<GEPI_FFUNC_FISHERPV><<<(unsigned)(t1 - t0), COUNT_THREADS, 0, p->stream>>>(ca);
+        const unsigned grid = (unsigned)(t1 - t0);
+        if (path == 2) {
+            if (entropy)
+                k_count_mfma<GEPI_FFUNC_ENTROPY><<<grid, MF_THREADS, 0, p->stream>>>(ca);
+            else
+                k_count_mfma<GEPI_FFUNC_FISHERPV><<<grid, MF_THREADS, 0, p->stream>>>(ca);
+        } else {
+            if (entropy)
+                k_count_popc<GEPI_FFUNC_ENTROPY><<<grid, POPC_THREADS, 0, p->stream>>>(ca);
+            else
+                k_count_popc<GEPI_FFUNC_FISHERPV><<<grid, POPC_THREADS, 0, p->stream>>>(ca);
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(p->ev1, p->stream));
-        HostState hs;
+        auto ts = std::chrono::steady_clock::now();
+        if (entropy)
+            k_exact<GEPI_FFUNC_ENTROPY><<<1024, 256, 0, p->stream>>>(ea);
+        else
+            k_exact<GEPI_FFUNC_FISHERPV><<<1024, 256, 0, p->stream>>>(ea);
+        HIP_TRY(hipGetLastError());
+        ScanState hs;
         if ((rc = read_state(p, &hs)) != GEPI_OK) return rc;
         float ms = 0;
         hipEventElapsedTime(&ms, p->ev0, p->ev1);
         ms_count += ms;
         bands++;
-        unsigned long long added = hs.count - count;
-        auto ts = std::chrono::steady_clock::now();
+        if (hs.pend_overflow) {
+            /* the pending list lost survivors; the candidates are untouched
+             * (k_exact skipped), so redo the band in smaller pieces        */
+            band = std::max<int64_t>(1, (t1 - t0) / 4);
+            ms_top += ms_since(ts);
+            continue;
+        }
+        exact_total += hs.pend_count;
+        const unsigned long long added = hs.count - count;
         if (hs.overflow) {
-            /* Threshold from what was kept, drop this band's records, redo it. */
-            unsigned long long kept = std::min<unsigned long long>(hs.count, cap);
-            ScanState fix;
-            HIP_TRY(hipMemcpyAsync(&fix, p->d_state, sizeof fix, hipMemcpyDeviceToHost, p->stream));
-            HIP_TRY(hipStreamSynchronize(p->stream));
-            fix.count = kept;
-            fix.overflow = 0;
-            HIP_TRY(hipMemcpyAsync(p->d_state, &fix, sizeof fix, hipMemcpyHostToDevice, p->stream));
+            /* candidates lost: threshold from what was kept, drop this band's
+             * records and redo the band                                     */
+            const unsigned long long kept = std::min<unsigned long long>(hs.count, cap);
+            hs.count = kept;
+            hs.overflow = 0;
+            HIP_TRY(hipMemcpyAsync(p->d_state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
             launch_select(p, p->d_cand[cur], (uint32_t)k, kept);
             k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
-                                                   t0, t1);
+                                                   t0, t1, tv);
             k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
+            HIP_TRY(hipGetLastError());
             cur ^= 1;
-            HostState h2;
+            ScanState h2;
             if ((rc = read_state(p, &h2)) != GEPI_OK) return rc;
             count = h2.count;
-            band = std::max<int64_t>(1, band / 4);
-            ms_sel += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts)
-                          .count();
-            continue; /* same t0 */
+            exact_total -= hs.pend_count;
+            band = std::max<int64_t>(1, (t1 - t0) / 4);
+            ms_top += ms_since(ts);
+            continue;
         }
         if (hs.count > (unsigned long long)k) {
             launch_select(p, p->d_cand[cur], (uint32_t)k, hs.count);
             k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
-                                                   0, 0);
+                                                   0, 0, tv);
             k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
             HIP_TRY(hipGetLastError());
             cur ^= 1;
@@ -889,31 +470,27 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         } else {
             count = hs.count;
         }
-        exact_total = hs.exact;
-        ms_sel +=
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
-        /* size the next band so its survivors fit comfortably */
-        int64_t tiles_done = t1 - t0;
-        double rate = (double)added / (double)tiles_done;
-        int64_t room = (int64_t)(cap - (unsigned long long)k) / 2;
-        int64_t next = rate > 0.0 ? (int64_t)((double)room / rate) : max_band;
+        ms_top += ms_since(ts);
+        /* size the next band so that its filter survivors fit comfortably */
+        const double rate = (double)std::max<unsigned long long>(hs.pend_count, added) /
+                            (double)(t1 - t0);
+        const int64_t room = (int64_t)(cap - (unsigned long long)k) / 2;
+        const int64_t next = rate > 0.0 ? (int64_t)((double)room / rate) : max_band;
         band = std::max<int64_t>(1, std::min<int64_t>({max_band, next, band * 4}));
         t0 = t1;
     }
-    /* final extraction: exactly k records */
+    /* final extraction: exactly k records, sorted by key on the host */
     auto ts = std::chrono::steady_clock::now();
     std::vector<Cand> host((size_t)count);
     if (count > 0)
         HIP_TRY(hipMemcpyAsync(host.data(), p->d_cand[cur], sizeof(Cand) * count,
                                hipMemcpyDeviceToHost, p->stream));
-    HostState hs;
-    if ((rc = read_state(p, &hs)) != GEPI_OK) return rc;
-    exact_total = hs.exact;
+    HIP_TRY(hipStreamSynchronize(p->stream));
     std::sort(host.begin(), host.end(), [](const Cand &a, const Cand &b) {
-        uint64_t ha = gepi::order_bits(a.score), hb = gepi::order_bits(b.score);
+        uint64_t ha = order_bits(a.score), hb = order_bits(b.score);
         return ha < hb || (ha == hb && a.idx < b.idx);
     });
-    int64_t n = std::min<int64_t>((int64_t)host.size(), k);
+    const int64_t n = std::min<int64_t>((int64_t)host.size(), k);
     for (int64_t i = 0; i < n; i++) {
         const Cand &c = host[(size_t)i];
         gepi_result &r = tops[i];
@@ -928,14 +505,12 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         r.reserved = 0;
     }
     *n_tops = n;
-    ms_sel += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
+    ms_top += ms_since(ts);
     st_local.ms_count_kernels = ms_count;
-    st_local.ms_select = ms_sel;
+    st_local.ms_select = ms_top;
     st_local.exact_evals = (int64_t)exact_total;
     st_local.bands = bands;
-    st_local.ms_scan =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
-            .count();
+    st_local.ms_scan = ms_since(t_start);
     if (n != k)
         return gepi_fail(GEPI_ERR_INTERNAL, "top-k extraction returned %lld of %lld records",
                          (long long)n, (long long)k);
@@ -1033,21 +608,21 @@ extern "C" int gepi_panel_create_bed(const uint8_t *bed_payload, int64_t n_varia
     const int64_t bytes = n_variants * bps;
     if (bytes > 0 && p->w64 > 0) {
         uint8_t *d_bed = nullptr;
-        /* pad so the decoder's last 4-sample read of a row stays in bounds */
-        if (hipMalloc(&d_bed, (size_t)bytes + 16) != hipSuccess) {
+        if (hipMalloc(&d_bed, (size_t)bytes) != hipSuccess) {
             panel_free(p);
             return gepi_fail(GEPI_ERR_NOMEM, "bed staging allocation failed");
         }
-        hipMemsetAsync(d_bed + bytes, 0, 16, p->stream);
         hipMemcpyAsync(d_bed, bed_payload, (size_t)bytes, hipMemcpyHostToDevice, p->stream);
         int64_t threads = n_variants * p->w64;
         k_decode_bed<<<(unsigned)((threads + 255) / 256), 256, 0, p->stream>>>(
             d_bed, n_variants, n_samples, p->w64, bps, p->d_nat);
         hipError_t e = hipStreamSynchronize(p->stream);
+        hipError_t e2 = hipGetLastError();
         hipFree(d_bed);
-        if (e != hipSuccess || hipGetLastError() != hipSuccess) {
+        if (e != hipSuccess || e2 != hipSuccess) {
             panel_free(p);
-            return gepi_fail(GEPI_ERR_DEVICE, "bed decode failed: %s", hipGetErrorString(e));
+            return gepi_fail(GEPI_ERR_DEVICE, "bed decode failed: %s",
+                             hipGetErrorString(e != hipSuccess ? e : e2));
         }
     }
     *out = p;

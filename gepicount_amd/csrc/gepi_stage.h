@@ -1,0 +1,102 @@
+/*
+ * gepi_stage.h -- panel staging kernels (replace the host transpose and
+ * uploads of src/kernels/setup1.h:72-102 and the constant-memory class masks
+ * of kernel1.cu:329-337).
+ *
+ * k_decode_bed     SNP-major .bed bytes -> [n_var][3][W64] planes on device
+ *                  (BinPlinkReader.h:171-201: code 0 -> allele 1 + valid,
+ *                  2 -> both + valid, 3 -> allele 2 + valid, 1 -> missing).
+ * k_gather_class   per scan: samples reordered class-contiguously
+ *                  (class 0 then class 1, each padded to 128 bits) into the
+ *                  popcount layout [n_var_pad][3][wc] dwords.
+ * k_transpose_k    popcount layout -> MFMA layout [n_ks][n_var_pad][3] of
+ *                  64-sample words, so that one K step of a tile is contiguous.
+ */
+#ifndef GEPI_STAGE_H
+#define GEPI_STAGE_H
+
+#include "gepi_device.h"
+
+namespace gepi {
+
+__global__ void k_decode_bed(const uint8_t *__restrict__ bed, int64_t n_var, int32_t n_samples,
+                             int32_t w64, int64_t bps, uint64_t *__restrict__ nat) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n_var * w64) return;
+    int64_t v = gid / w64;
+    int32_t wd = (int32_t)(gid % w64);
+    const uint8_t *src = bed + v * bps;
+    uint64_t a1 = 0, a2 = 0, vv = 0;
+    int32_t s0 = wd * 64;
+    int32_t s1 = s0 + 64 < n_samples ? s0 + 64 : n_samples;
+    for (int32_t s = s0; s < s1; s += 4) {
+        uint32_t byte = src[s >> 2];
+        for (int32_t q = 0; q < 4 && s + q < s1; q++) {
+            uint32_t code = (byte >> (2 * q)) & 3u;
+            uint64_t bit = 1ull << ((s + q) & 63);
+            if (code == 1u) continue;
+            vv |= bit;
+            if (code != 3u) a1 |= bit;
+            if (code != 0u) a2 |= bit;
+        }
+    }
+    uint64_t *dst = nat + (v * 3) * w64 + wd;
+    dst[0] = a1;
+    dst[w64] = a2;
+    dst[2 * w64] = vv;
+}
+
+/* out[v][p][d] bit b = panel bit perm[32*d + b] (0 for perm < 0 or v >= n_var). */
+__global__ void __launch_bounds__(256) k_gather_class(const uint64_t *__restrict__ nat,
+                                                       int32_t n_var, int32_t w64,
+                                                       const int32_t *__restrict__ perm,
+                                                       int32_t wc, int32_t wused,
+                                                       uint32_t *__restrict__ out) {
+    extern __shared__ uint64_t srow[];
+    const int32_t v = blockIdx.x;
+    const bool real = v < n_var;
+    for (int32_t i = threadIdx.x; i < 3 * w64; i += blockDim.x)
+        srow[i] = real ? nat[(int64_t)v * 3 * w64 + i] : 0ull;
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < 3 * wc; i += blockDim.x) {
+        int32_t p = i / wc, d = i - p * wc;
+        uint32_t word = 0;
+        if (real && d < wused) {
+            const int32_t *pp = perm + 32 * d;
+            const uint64_t *row = srow + p * w64;
+#pragma unroll 8
+            for (int32_t b = 0; b < 32; b++) {
+                int32_t s = pp[b];
+                if (s >= 0) word |= (uint32_t)((row[s >> 6] >> (s & 63)) & 1ull) << b;
+            }
+        }
+        out[((int64_t)v * 3 + p) * wc + d] = word;
+    }
+}
+
+/* Tile of 32 variants x 32 K steps per workgroup. */
+__global__ void __launch_bounds__(256) k_transpose_k(const uint32_t *__restrict__ cls,
+                                                      int32_t n_var_pad, int32_t wc, int32_t n_ks,
+                                                      uint64_t *__restrict__ out) {
+    __shared__ uint64_t tile[96][33];
+    const int32_t v0 = blockIdx.x * 32, s0 = blockIdx.y * 32;
+    for (int32_t idx = threadIdx.x; idx < 96 * 32; idx += blockDim.x) {
+        const int32_t vp = idx >> 5, k = idx & 31;
+        const int32_t s = s0 + k;
+        const int32_t v = v0 + vp / 3, p = vp % 3;
+        uint64_t val = 0;
+        if (s < n_ks)
+            val = *reinterpret_cast<const uint64_t *>(cls + ((int64_t)v * 3 + p) * wc + 2 * s);
+        tile[vp][k] = val;
+    }
+    __syncthreads();
+    for (int32_t idx = threadIdx.x; idx < 96 * 32; idx += blockDim.x) {
+        const int32_t k = idx / 96, vp = idx - 96 * k;
+        const int32_t s = s0 + k;
+        if (s < n_ks) out[((int64_t)s * n_var_pad + v0) * 3 + vp] = tile[vp][k];
+    }
+}
+
+} // namespace gepi
+
+#endif

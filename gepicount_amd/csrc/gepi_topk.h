@@ -1,0 +1,250 @@
+/*
+ * gepi_topk.h -- exact scoring of filter survivors and the on-device
+ * streaming top-k (replaces k1_entropy / k1_fisherpv, src/kernels/entropyD.cu:31-50,
+ * src/kernels/fisher_d.cu:290-311, the per-tile thrust::sort_by_key of
+ * kernel1.cu:448 and the host merge of misc1.cu:234-264).
+ *
+ * Keys are (order_bits(score), idx) as 128-bit unsigned values; idx orders
+ * like (sA, sB, fun), so the key order is the canonical (score, sA, sB, fun).
+ * The select is an MSB-first radix select with 8-bit digits that finishes
+ * with a bitonic sort in LDS once at most SEL_SMALL keys share the prefix.
+ */
+#ifndef GEPI_TOPK_H
+#define GEPI_TOPK_H
+
+#include "gepi_device.h"
+
+namespace gepi {
+
+struct ExactArgs {
+    ScanState *st;
+    const Pend *pend;
+    unsigned long long pcap;
+    Cand *cand;
+    unsigned long long cap;
+    const double *lhi, *llo;
+};
+
+/* Exact score of every pending evaluation; admits keys <= the threshold. */
+template <int FFUN>
+__global__ void __launch_bounds__(256) k_exact(ExactArgs e) {
+    if (e.st->pend_overflow) return;
+    unsigned long long n = e.st->pend_count;
+    if (n > e.pcap) n = e.pcap;
+    const double thr_s = e.st->thr_score;
+    const uint64_t thr_i = e.st->thr_idx;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const Pend p = e.pend[i];
+        const int32_t w = p.w, x = p.x, y = p.y, z = p.z;
+        const double s = (FFUN == GEPI_FFUNC_ENTROPY) ? entropy_exact(w, x, y, z, e.lhi, e.llo)
+                                                      : fisher_exact(w, x, y, z);
+        if (s < thr_s || (s == thr_s && p.idx <= thr_i)) {
+            unsigned long long pos = atomicAdd(&e.st->count, 1ull);
+            if (pos < e.cap) {
+                Cand c;
+                c.score = s;
+                c.idx = p.idx;
+                c.w = (uint32_t)w;
+                c.x = (uint32_t)x;
+                c.y = (uint32_t)y;
+                c.z = (uint32_t)z;
+                e.cand[pos] = c;
+            } else {
+                atomicOr(&e.st->overflow, 1u);
+            }
+        }
+    }
+}
+
+/* Clears the pending list at the start of a band. */
+__global__ void k_reset_pending(ScanState *st) {
+    if (threadIdx.x != 0) return;
+    st->pend_count = 0;
+    st->pend_overflow = 0;
+}
+
+struct SelPrefix {
+    uint64_t key_hi, key_lo;
+    uint32_t bits_done;
+};
+
+__device__ __forceinline__ SelPrefix load_prefix(const ScanState *st) {
+    SelPrefix s;
+    s.key_hi = st->key_hi;
+    s.key_lo = st->key_lo;
+    s.bits_done = st->bits_done;
+    return s;
+}
+
+__device__ __forceinline__ bool key_match(uint64_t hi, uint64_t lo, const SelPrefix &s) {
+    uint32_t b = s.bits_done;
+    if (b == 0) return true;
+    if (b < 64) return (hi >> (64 - b)) == (s.key_hi >> (64 - b));
+    if (hi != s.key_hi) return false;
+    if (b == 64) return true;
+    return (lo >> (128 - b)) == (s.key_lo >> (128 - b));
+}
+
+__device__ __forceinline__ uint32_t key_digit(uint64_t hi, uint64_t lo, uint32_t b) {
+    return b < 64 ? (uint32_t)((hi >> (56 - b)) & 0xFF) : (uint32_t)((lo >> (120 - b)) & 0xFF);
+}
+
+__global__ void __launch_bounds__(256) k_sel_hist(const Cand *__restrict__ cand, ScanState *st) {
+    if (st->done) return;
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const SelPrefix s = load_prefix(st);
+    const unsigned long long n = st->count;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        uint64_t hi = order_bits(cand[i].score), lo = cand[i].idx;
+        if (key_match(hi, lo, s)) atomicAdd(&h[key_digit(hi, lo, s.bits_done)], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_sel_pick(ScanState *st) {
+    if (threadIdx.x != 0 || st->done) return;
+    uint32_t cum = 0, d = 0;
+    for (d = 0; d < 256; d++) {
+        uint32_t c = st->hist[d];
+        if (cum + c >= st->rank) break;
+        cum += c;
+    }
+    if (d == 256) { /* rank beyond the matching records: cannot happen for rank <= count */
+        st->done = 3;
+        return;
+    }
+    uint32_t b = st->bits_done;
+    if (b < 64)
+        st->key_hi |= (uint64_t)d << (56 - b);
+    else
+        st->key_lo |= (uint64_t)d << (120 - b);
+    st->rank -= cum;
+    st->matched = st->hist[d];
+    st->bits_done = b + 8;
+    for (int i = 0; i < 256; i++) st->hist[i] = 0;
+    if (st->bits_done == 128)
+        st->done = 1;
+    else if (st->matched <= SEL_SMALL)
+        st->done = 2;
+}
+
+__global__ void __launch_bounds__(256) k_sel_gather(const Cand *__restrict__ cand, ScanState *st,
+                                                     uint64_t *small) {
+    if (st->done != 2) return;
+    const SelPrefix s = load_prefix(st);
+    const unsigned long long n = st->count;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        uint64_t hi = order_bits(cand[i].score), lo = cand[i].idx;
+        if (key_match(hi, lo, s)) {
+            uint32_t p = atomicAdd(&st->gather_n, 1u);
+            if (p < SEL_SMALL) {
+                small[2 * p] = hi;
+                small[2 * p + 1] = lo;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_sel_small(ScanState *st, const uint64_t *small) {
+    if (st->done != 2) return;
+    __shared__ uint64_t kh[SEL_SMALL], kl[SEL_SMALL];
+    const uint32_t n = st->gather_n < SEL_SMALL ? st->gather_n : SEL_SMALL;
+    const int t = threadIdx.x;
+    kh[t] = t < (int)n ? small[2 * t] : ~0ull;
+    kl[t] = t < (int)n ? small[2 * t + 1] : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= SEL_SMALL; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            int partner = t ^ stride;
+            if (partner > t) {
+                bool up = (t & size) == 0;
+                bool gt = kh[t] > kh[partner] || (kh[t] == kh[partner] && kl[t] > kl[partner]);
+                if (gt == up) {
+                    uint64_t a = kh[t], b = kl[t];
+                    kh[t] = kh[partner];
+                    kl[t] = kl[partner];
+                    kh[partner] = a;
+                    kl[partner] = b;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t == 0) {
+        uint32_t r = st->rank;
+        st->key_hi = kh[r - 1];
+        st->key_lo = kl[r - 1];
+        st->bits_done = 128;
+        st->done = 1;
+    }
+}
+
+/* Keeps records with key <= the selected key; records of tiles in
+ * [drop_t0, drop_t1) (tile side tv) are dropped as well.                    */
+__global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, ScanState *st,
+                                                  Cand *__restrict__ dst, int64_t drop_t0,
+                                                  int64_t drop_t1, int32_t tv) {
+    const uint64_t khi = st->key_hi, klo = st->key_lo;
+    const unsigned long long n = st->count;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        Cand c = src[i];
+        uint64_t hi = order_bits(c.score);
+        bool keep = hi < khi || (hi == khi && c.idx <= klo);
+        if (keep && drop_t1 > drop_t0) {
+            int64_t a = (int64_t)(c.idx >> 32) >> 1;
+            int64_t b = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull) >> 1;
+            int64_t ta = a / tv, tb = b / tv;
+            int64_t t = ta * (ta + 1) / 2 + tb;
+            keep = t < drop_t0 || t >= drop_t1;
+        }
+        if (keep) {
+            unsigned long long p = atomicAdd(&st->count2, 1ull);
+            dst[p] = c;
+        }
+    }
+}
+
+/* After a compaction: adopt the compacted buffer and the new threshold. */
+__global__ void k_finalize(ScanState *st, int32_t ffun) {
+    if (threadIdx.x != 0) return;
+    st->count = st->count2;
+    st->count2 = 0;
+    st->overflow = 0;
+    double thr = from_order_bits(st->key_hi);
+    st->thr_score = thr;
+    st->thr_idx = st->key_lo;
+    float kf = 0.0f;
+    if (ffun == GEPI_FFUNC_ENTROPY) {
+        double it = -thr - 1e-12;
+        if (it > 0.0) {
+            double k = expm1(it * 0.6931471805599453) * (1.0 - 1e-6);
+            kf = (float)k;
+            if ((double)kf > k) kf = nextafterf(kf, 0.0f);
+        }
+    }
+    st->filt_k = kf;
+}
+
+/* Resets the select state for a new select of the rank-th smallest key. */
+__global__ void k_sel_reset(ScanState *st, uint32_t rank) {
+    if (threadIdx.x != 0) return;
+    st->key_hi = 0;
+    st->key_lo = 0;
+    st->bits_done = 0;
+    st->rank = rank;
+    st->matched = 0;
+    st->done = 0;
+    st->gather_n = 0;
+    for (int i = 0; i < 256; i++) st->hist[i] = 0;
+}
+
+} // namespace gepi
+
+#endif

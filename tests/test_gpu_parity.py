@@ -39,40 +39,48 @@ def random_panel(seed, n_var, n, miss, cls_p=(0.5, 0.45, 0.05)):
     return bits, valid, m0, m1
 
 
+PATHS = [1, 2]  # 1 = popcount (VALU), 2 = int8 MFMA
+
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("seed,n_var,n,miss", [
     (1, 1, 7, 0.0), (2, 5, 64, 0.0), (3, 33, 65, 0.05), (4, 40, 200, 0.1), (5, 70, 129, 0.02),
-    (6, 32, 300, 0.0)])
-def test_all_evaluations_entropy(seed, n_var, n, miss):
-    print("case seed=%d n_var=%d n=%d miss=%g" % (seed, n_var, n, miss), flush=True)
+    (6, 32, 300, 0.0), (7, 130, 70, 0.03)])
+def test_all_evaluations_entropy(seed, n_var, n, miss, path):
+    print("case seed=%d n_var=%d n=%d miss=%g path=%d" % (seed, n_var, n, miss, path), flush=True)
     bits, valid, m0, m1 = random_panel(seed, n_var, n, miss)
     R = 2 * n_var
     total = 5 * R * (R + 1) // 2
-    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=total)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=total, count_path=path)
+    assert st["count_path"] == path
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), total)
     assert_same(got, ref, "all-evals")
     assert st["evals"] == total
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("seed,n_var,n,miss,k", [
     (11, 150, 400, 0.02, 50), (12, 200, 700, 0.0, 500), (13, 97, 1000, 0.05, 7),
     (14, 260, 256, 0.01, 3000)])
-def test_topk_entropy_threshold_active(seed, n_var, n, miss, k):
-    print("case seed=%d n_var=%d n=%d k=%d" % (seed, n_var, n, k), flush=True)
+def test_topk_entropy_threshold_active(seed, n_var, n, miss, k, path):
+    print("case seed=%d n_var=%d n=%d k=%d path=%d" % (seed, n_var, n, k, path), flush=True)
     bits, valid, m0, m1 = random_panel(seed, n_var, n, miss)
-    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k, count_path=path)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), k)
     assert_same(got, ref, "topk")
     # the filter must have pruned most evaluations
     assert st["exact_evals"] < st["evals"]
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("seed,n_var,n,miss,k", [(21, 12, 90, 0.05, None), (22, 60, 300, 0.02, 40)])
-def test_fisher_objective(seed, n_var, n, miss, k):
-    print("case fisher seed=%d n_var=%d n=%d" % (seed, n_var, n), flush=True)
+def test_fisher_objective(seed, n_var, n, miss, k, path):
+    print("case fisher seed=%d n_var=%d n=%d path=%d" % (seed, n_var, n, path), flush=True)
     bits, valid, m0, m1 = random_panel(seed, n_var, n, miss)
     R = 2 * n_var
     kk = k or 5 * R * (R + 1) // 2
-    got, _ = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=kk, ffun=ge.GEPI_FFUNC_FISHERPV)
+    got, _ = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=kk, ffun=ge.GEPI_FFUNC_FISHERPV,
+                              count_path=path)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), kk, ffun=1)
     assert_same(got, ref, "fisher")
 
@@ -104,9 +112,10 @@ def test_small_workload_top100_both_objectives():
         bits, valid = panel.download()
         op = orc.Panel(bits, valid, m0, m1, p["n_samples"])
         for ffun in (0, 1):
-            got, _ = panel.scan(m0, m1, topmost=100, ffun=ffun)
             ref = orc.gold_scan(op, 100, ffun=ffun)
-            assert_same(got, ref, "small ffun=%d" % ffun)
+            for path in PATHS:
+                got, _ = panel.scan(m0, m1, topmost=100, ffun=ffun, count_path=path)
+                assert_same(got, ref, "small ffun=%d path=%d" % (ffun, path))
 
 
 def test_medium_subset_top1000():
@@ -114,10 +123,11 @@ def test_medium_subset_top1000():
     p = synth.make_panel("medium", n_var=2000)
     m0, m1 = synth.masks_from_pheno(p["pheno"])
     with ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"]) as panel:
-        got, st = panel.scan(m0, m1, topmost=1000)
         bits, valid = panel.download()
-    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, p["n_samples"]), 1000)
-    assert_same(got, ref, "medium-2000")
+        ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, p["n_samples"]), 1000)
+        for path in PATHS:
+            got, st = panel.scan(m0, m1, topmost=1000, count_path=path)
+            assert_same(got, ref, "medium-2000 path=%d" % path)
     # the planted XOR pairs dominate the ranking
     top_pairs = {(int(r["sA"]) // 2, int(r["sB"]) // 2) for r in got[:50]}
     planted = {(max(a, b), min(a, b)) for a, b in p["planted"]}
@@ -153,14 +163,43 @@ def test_overlapping_and_empty_masks():
         assert_same(got, ref, "masks")
 
 
-def test_overflow_recovery_path(monkeypatch):
-    """A tiny candidate buffer forces the band to overflow and be redone."""
-    bits, valid, m0, m1 = random_panel(51, 300, 300, 0.01)
-    monkeypatch.setenv("GEPI_CAND_CAP", "1")
-    got, st = ge.process_gpu_1(bits, valid, m0, m1, 300, topmost=100)
-    monkeypatch.delenv("GEPI_CAND_CAP")
+@pytest.mark.parametrize("path", PATHS)
+def test_overflow_recovery_path(monkeypatch, path):
+    """A small pending/candidate buffer forces bands to overflow and be redone."""
+    bits, valid, m0, m1 = random_panel(51, 500, 300, 0.01)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 300), 100)
+    monkeypatch.setenv("GEPI_CAND_CAP", "1")
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, 300, topmost=100, count_path=path)
+    monkeypatch.delenv("GEPI_CAND_CAP")
     assert_same(got, ref, "overflow")
+
+
+def test_mfma_class_encoding_limits():
+    """The MFMA path keeps one class in the low 14 bits of its accumulators
+    (at most 16383 samples there).  A class above that puts the other class
+    in the low bits; when both exceed it, the automatic path is popcount and
+    an explicit MFMA request fails with GEPI_ERR_LIMIT."""
+    rng = np.random.default_rng(71)
+    n = 16600
+    codes = orc.random_codes(rng, 3, n, 0.01)
+    bits, valid = orc.planes_from_codes(codes)
+    cls = np.where(np.arange(n) < 16400, 0, 1)      # 16400 controls, 200 cases
+    m0, m1 = orc.masks_from_classes(cls)
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), 10**6)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=10**6, count_path=2)
+    assert_same(got, ref, "mfma-big-class")
+    n = 33000
+    codes = orc.random_codes(rng, 3, n, 0.01)
+    bits, valid = orc.planes_from_codes(codes)
+    cls = np.where(np.arange(n) < 16500, 0, 1)      # both classes > 16383
+    m0, m1 = orc.masks_from_classes(cls)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=10**6)
+    assert st["count_path"] == 1
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), 10**6)
+    assert_same(got, ref, "popcount-fallback")
+    with pytest.raises(ge.GepiError) as e:
+        ge.process_gpu_1(bits, valid, m0, m1, n, topmost=10, count_path=2)
+    assert e.value.code == -3
 
 
 def test_topk_zero_and_more_than_all():
