@@ -138,24 +138,15 @@ constexpr int MF_THREADS = 256;  /* 4 waves, 2 x 2 of 32 x 32 variants          
 constexpr int MF_ROWS = 2 * TM * 3; /* LDS rows per K step: A then B, 3 planes  */
 constexpr int MF_ROW = 80;       /* bytes per LDS row: 64 samples + 16 pad      */
 
-/* 64 packed samples -> 64 bytes (0x01 or 0x80 per set bit) at an LDS row.
- * Nibble n spreads to bytes with (n * 0x204081) & 0x01010101.              */
-__device__ __forceinline__ void expand_row(uint8_t *__restrict__ row, uint64_t bits, bool high) {
-    const uint32_t lo = (uint32_t)bits, hi = (uint32_t)(bits >> 32);
-    uint32_t d[16];
+/* 32 packed samples -> 32 bytes at an LDS half row: bit b becomes byte b,
+ * 0x01 (shift 0) or 0x80 (shift 7).  Nibble n spreads with
+ * (n * 0x204081) & 0x01010101.                                            */
+__device__ __forceinline__ void expand_half(uint8_t *__restrict__ dst, uint32_t bits, int shift) {
+    uint32_t d[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        d[i] = (((lo >> (4 * i)) & 0xFu) * 0x204081u) & 0x01010101u;
-        d[8 + i] = (((hi >> (4 * i)) & 0xFu) * 0x204081u) & 0x01010101u;
-    }
-    if (high) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) d[i] <<= 7;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-        *reinterpret_cast<uint4 *>(row + 16 * i) =
-            make_uint4(d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
+    for (int i = 0; i < 8; i++) d[i] = ((((bits >> (4 * i)) & 0xFu) * 0x204081u) & 0x01010101u) << shift;
+    *reinterpret_cast<uint4 *>(dst) = make_uint4(d[0], d[1], d[2], d[3]);
+    *reinterpret_cast<uint4 *>(dst + 16) = make_uint4(d[4], d[5], d[6], d[7]);
 }
 
 template <int FFUN>
@@ -179,33 +170,46 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
 #pragma unroll
                 for (int q = 0; q < 3; q++) acc[i][j][p][q] = v4i{0, 0, 0, 0};
 
-    /* staging: element e of a K step is row e of the LDS buffer; e < 192 is
-     * (A variant e/3, plane e%3), e >= 192 is B variant (e-192)/3.          */
-    const uint64_t *srcA = g.planes_k + (int64_t)ta * TM * 3;
-    const uint64_t *srcB = g.planes_k + (int64_t)tb * TM * 3;
-    const int64_t sstride = (int64_t)g.n_var_pad * 3;
-    const uint64_t *src0 = tid < 192 ? srcA + tid : srcB + (tid - 192);
-    const uint64_t *src1 = srcB + (tid + 64); /* element tid + 256, for tid < 128 */
-    uint64_t r0 = 0, r1 = 0;
+    /* Staging.  A K step holds 384 LDS rows of 64 samples: row e < 192 is A
+     * variant e/3 plane e%3, row e >= 192 is B variant (e-192)/3.  Thread t
+     * expands half rows h = t, t+256, t+512 (row h/2, samples 32*(h%2)..):
+     * the 32-bit word at dword offset h of the step's A or B block.         */
+    const uint32_t *srcA = reinterpret_cast<const uint32_t *>(g.planes_k) + (int64_t)ta * TM * 6;
+    const uint32_t *srcB = reinterpret_cast<const uint32_t *>(g.planes_k) + (int64_t)tb * TM * 6;
+    const int64_t sstride = (int64_t)g.n_var_pad * 6; /* dwords per K step */
+    const uint32_t *src[3];
+    int dst[3];
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+        const int h = tid + 256 * u;                  /* 0..767 */
+        src[u] = h < 384 ? srcA + h : srcB + (h - 384);
+        dst[u] = (h >> 1) * MF_ROW + 32 * (h & 1);
+    }
+    const int last = g.n_ks - 1;
+    uint32_t pf1[3], pf2[3];
+    auto shift_of = [&](int s) { return ((s < g.ks0) == (g.cls0_high != 0)) ? 7 : 0; };
 
-    r0 = src0[0];
-    if (tid < 128) r1 = src1[0];
     {
-        const bool high = (0 < g.ks0) ? g.cls0_high : !g.cls0_high;
-        expand_row(&lds[0][tid * MF_ROW], r0, high);
-        if (tid < 128) expand_row(&lds[0][(tid + 256) * MF_ROW], r1, high);
+        uint32_t r[3];
+#pragma unroll
+        for (int u = 0; u < 3; u++) r[u] = src[u][0];
+        const int sh = shift_of(0);
+#pragma unroll
+        for (int u = 0; u < 3; u++) expand_half(&lds[0][dst[u]], r[u], sh);
+        const int64_t o1 = (int64_t)(last < 1 ? last : 1) * sstride;
+        const int64_t o2 = (int64_t)(last < 2 ? last : 2) * sstride;
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            pf1[u] = src[u][o1];
+            pf2[u] = src[u][o2];
+        }
     }
     __syncthreads();
 
     for (int32_t s = 0; s < g.n_ks; s++) {
         const int buf = s & 1;
-        const bool more = s + 1 < g.n_ks;
-        if (more) {
-            r0 = src0[(s + 1) * sstride];
-            if (tid < 128) r1 = src1[(s + 1) * sstride];
-        }
-        v4i a[2][3], b[2][3];
         const uint8_t *L = lds[buf];
+        v4i a[2][3], b[2][3];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -215,6 +219,12 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
                 b[i][p] = *reinterpret_cast<const v4i *>(
                     L + (192 + (32 * wb + 16 * i + l16) * 3 + p) * MF_ROW + 16 * lg);
             }
+        /* next step's expansion (harmless extra copy on the last step: the
+         * other buffer is not read again) interleaved with this step's MFMAs */
+        const int sh = shift_of(s + 1);
+        uint8_t *W = lds[buf ^ 1];
+#pragma unroll
+        for (int u = 0; u < 3; u++) expand_half(&W[dst[u]], pf1[u], sh);
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -225,11 +235,18 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
                     for (int q = 0; q < 3; q++)
                         acc[i][j][p][q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
                             a[i][p], b[j][q], acc[i][j][p][q], 0, 0, 0);
-        if (more) {
-            const bool high = (s + 1 < g.ks0) ? g.cls0_high : !g.cls0_high;
-            expand_row(&lds[buf ^ 1][tid * MF_ROW], r0, high);
-            if (tid < 128) expand_row(&lds[buf ^ 1][(tid + 256) * MF_ROW], r1, high);
+        const int64_t on = (int64_t)(s + 3 <= last ? s + 3 : last) * sstride;
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            pf1[u] = pf2[u];
+            pf2[u] = src[u][on];
         }
+#pragma unroll
+        for (int k = 0; k < 36; k++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* 1 MFMA      */
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0); /* 3 VALU      */
+        }
+        __builtin_amdgcn_sched_group_barrier(0x200, 6, 0);     /* DS writes   */
         __syncthreads();
     }
 
