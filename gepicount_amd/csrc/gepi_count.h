@@ -136,7 +136,7 @@ typedef int32_t v4i __attribute__((ext_vector_type(4)));
 
 constexpr int MF_THREADS = 256;  /* 4 waves, 2 x 2 of 32 x 32 variants          */
 constexpr int MF_ROWS = 2 * TM * 3; /* LDS rows per K step: A then B, 3 planes  */
-constexpr int MF_ROW = 80;       /* bytes per LDS row: 64 samples + 16 pad      */
+constexpr int MF_ROW = 64;       /* bytes per LDS row: 64 samples, unpadded     */
 
 /* 32 packed samples -> 32 bytes at an LDS half row: bit b becomes byte b,
  * 0x01 (shift 0) or 0x80 (shift 7).  Nibble n spreads with

@@ -126,17 +126,9 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, const FilterCons
     const int32_t y = c[0][2][2], z = c[1][2][2];
     const int32_t m = y + z;
     const float kyz = fc.kf * (float)(y * z) * (1.0f - 1.0f / 65536.0f);
-    /* row pairs (2A+ia, 2B+ib); on the diagonal (2A, 2A+1) is not a pair.
-     * ROLLED: a short runtime loop keeps the unrolled code of callers with
-     * many pairs per thread small (values picked by selects).               */
-    auto row_pair = [&](int rp) {
-        const int ia = rp >> 1, ib = rp & 1;
-        if (A == B && ia < ib) return;
-        const bool i1 = ia != 0, j1 = ib != 0;
-        const int32_t n0 = i1 ? (j1 ? c[0][1][1] : c[0][1][0]) : (j1 ? c[0][0][1] : c[0][0][0]);
-        const int32_t n1 = i1 ? (j1 ? c[1][1][1] : c[1][1][0]) : (j1 ? c[1][0][1] : c[1][0][0]);
-        const int32_t a0 = i1 ? c[0][1][2] : c[0][0][2], a1 = i1 ? c[1][1][2] : c[1][0][2];
-        const int32_t b0 = j1 ? c[0][2][1] : c[0][2][0], b1 = j1 ? c[1][2][1] : c[1][2][0];
+    /* row pairs (2A+ia, 2B+ib); on the diagonal (2A, 2A+1) is not a pair.  */
+    auto row_pair = [&](int ia, int ib, int32_t n0, int32_t n1, int32_t a0, int32_t a1, int32_t b0,
+                        int32_t b1) {
         const int32_t ws[5] = {n0, b0 - n0, a0 - n0, y - a0 - b0 + n0, a0 + b0 - 2 * n0};
         const int32_t xs[5] = {n1, b1 - n1, a1 - n1, z - a1 - b1 + n1, a1 + b1 - 2 * n1};
         const uint64_t base = pack_idx(2 * (int64_t)A + ia, 2 * (int64_t)B + ib, 0);
@@ -163,11 +155,32 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, const FilterCons
         }
     };
     if (ROLLED) {
+        /* A short runtime loop keeps the code of callers with many pairs per
+         * thread small.  The per-row-pair operands advance through rotating
+         * registers (no indexed array, which the compiler would place in
+         * scratch memory): row pair rp = 2*ia + ib.                        */
+        int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
+        int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
+        int32_t a0a = c[0][0][2], a0b = c[0][1][2], a1a = c[1][0][2], a1b = c[1][1][2];
+        int32_t b0a = c[0][2][0], b0b = c[0][2][1], b1a = c[1][2][0], b1b = c[1][2][1];
 #pragma unroll 1
-        for (int rp = 0; rp < 4; rp++) row_pair(rp);
+        for (int rp = 0; rp < 4; rp++) {
+            const int ia = rp >> 1, ib = rp & 1;
+            if (!(A == B && ia < ib)) row_pair(ia, ib, n0a, n1a, a0a, a1a, b0a, b1a);
+            n0a = n0b; n0b = n0c; n0c = n0d;
+            n1a = n1b; n1b = n1c; n1c = n1d;
+            if (ib) { a0a = a0b; a1a = a1b; }
+            int32_t t = b0a; b0a = b0b; b0b = t;
+            t = b1a; b1a = b1b; b1b = t;
+        }
     } else {
 #pragma unroll
-        for (int rp = 0; rp < 4; rp++) row_pair(rp);
+        for (int rp = 0; rp < 4; rp++) {
+            const int ia = rp >> 1, ib = rp & 1;
+            if (A == B && ia < ib) continue;
+            row_pair(ia, ib, c[0][ia][ib], c[1][ia][ib], c[0][ia][2], c[1][ia][2], c[0][2][ib],
+                     c[1][2][ib]);
+        }
     }
 }
 
