@@ -8,8 +8,9 @@ top-k to the host.  The workload is the 100,000-variant x 8,192-sample panel
 Prints one JSON line (rank 0, N = 1):
   value       evals/s = 5 * R(R+1)/2 / mean step seconds, R = 2 x variants
   roofline    the count kernel: algorithmic ops per launch / mean launch time
-              (HIP events on the scan stream) against the MI355X dense int8
-              MFMA peak (MFMA path) or the gfx950 VALU peak (popcount path)
+              (HIP events on the scan stream) against the MI355X dense fp4
+              MFMA peak (fp4 path, the default), the dense int8 MFMA peak
+              (int8 path) or the gfx950 VALU peak (popcount path)
   cpu_baseline  the oracle's OpenMP gold scan on a bounded leading subset of
               the same panel, on this host's cores
 """
@@ -28,6 +29,8 @@ METRIC = "SNP-pair x Boolean-fn evals/s at 100k SNPs x 8k samples; p50/p99 scan 
 VALU_PEAK_OPS = 256 * 64 * 2.4e9
 # MI355X dense int8 matrix peak
 INT8_MFMA_PEAK_OPS = 5.0e15
+# MI355X dense fp4 (e2m1) matrix peak
+FP4_MFMA_PEAK_FLOPS = 10.1e15
 
 
 def parse():
@@ -125,14 +128,18 @@ def main():
     total = time.perf_counter() - t0
     ms_per_step = 1000.0 * total / args.steps
 
-    # count kernel roofline.  Both paths form 9 plane products per variant pair
+    # count kernel roofline.  All paths form 9 plane products per variant pair
     # over the n class samples.  Popcount: 2 VALU ops (and, bcnt) per 32-bit
-    # word of each product.  MFMA: 9 x n int8 MACs = 18 n ops per variant pair.
+    # word of each product.  MFMA: 9 x n MACs = 18 n ops per variant pair
+    # (int8 operands, or fp4 operands with fp32 accumulation).
     var_pairs = n_var * (n_var + 1) // 2
     path = stats[-1]["count_path"]
     k_ms = statistics.mean(s["ms_count_kernels"] for s in stats)
     launches = statistics.mean(s["bands"] for s in stats)
-    if path == 2:
+    if path == 3:
+        ops_total = var_pairs * 9 * 2.0 * (n0 + n1)
+        peak, bound, unit, kernel = FP4_MFMA_PEAK_FLOPS, "mfma", "TFLOP/s", "k_count_f4"
+    elif path == 2:
         ops_total = var_pairs * 9 * 2.0 * (n0 + n1)
         peak, bound, unit, kernel = INT8_MFMA_PEAK_OPS, "mfma", "TOP/s", "k_count_mfma"
     else:
@@ -159,7 +166,7 @@ def main():
         "metric": METRIC, "value": evals / statistics.mean(times), "unit": "evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "i8" if path == 2 else "u32",
+        "dtype": {3: "fp4 (e2m1, fp32 accumulate)", 2: "i8"}.get(path, "u32"),
         "data": "synthetic",
         "config": {"workload": "%s: %d variants x %d samples (%d cases / %d controls), top-%d, %s"
                                % (args.workload, n_var, n, n1, n0, args.topk,

@@ -16,6 +16,14 @@
  *               16383): both classes in one pass over K.  64x64-variant
  *               workgroup tiles, 4 waves of 32x32, 64-sample K steps double
  *               buffered in LDS.
+ * k_count_f4    matrix-core path on fp4 (e2m1) operands: packed bits staged
+ *               in LDS, each wave expands its own fragments in registers (one
+ *               bit -> one 0.5-valued nibble, 7 VALU ops per 32 samples) for
+ *               v_mfma_scale_f32_16x16x128_f8f6f4 with block scale 2, so a
+ *               product is exactly 1.0 and the fp32 accumulators hold exact
+ *               counts (< 2^24).  Classes use separate accumulators (no class
+ *               size limit).  64x32-variant workgroup tiles, 4 waves of 32x16,
+ *               128-sample K steps double buffered in LDS.
  */
 #ifndef GEPI_COUNT_H
 #define GEPI_COUNT_H
@@ -277,6 +285,153 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
                     }
                 filter_pair<FFUN, true>(g, fc, A, B, c);
             }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* fp4 MFMA path                                                           */
+
+typedef int32_t v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int F4_THREADS = 256; /* 4 waves, 2 x 2 of 32 x 16 variants             */
+constexpr int TB4 = 32;         /* B variants per workgroup tile (A: TM = 64)     */
+constexpr int F4_QA = TM * 3;   /* uint4 (128-sample rows) of A per K step        */
+constexpr int F4_Q = F4_QA + TB4 * 3; /* uint4 per K step: A rows then B rows    */
+constexpr int F4_FMT = 4;       /* operand format code of e2m1                    */
+constexpr int F4_SCALE = 128;   /* E8M0 block scale 2^1: 0.5 * 2 = 1.0           */
+
+/* Workgroup tile t = ta*(ta+1) + tb, 0 <= tb <= 2*ta+1: A variants
+ * [64ta, 64ta+64) against B variants [32tb, 32tb+32). */
+__host__ __device__ inline void tile_of_f4(int64_t t, int32_t &ta, int32_t &tb) {
+    double r = sqrt(4.0 * (double)t + 1.0);
+    int64_t a = (int64_t)((r - 1.0) * 0.5);
+    while (a * (a + 1) > t) a--;
+    while ((a + 1) * (a + 2) <= t) a++;
+    ta = (int32_t)a;
+    tb = (int32_t)(t - a * (a + 1));
+}
+
+/* 32 packed samples -> the 4 dwords of one lane's fp4 fragment: nibble n of
+ * dword d is bit 4n+d of w (0x1 = e2m1 0.5, 0x0 = 0).  The order of samples
+ * within a K step does not change a dot product, and A and B fragments use
+ * the same order.                                                          */
+__device__ __forceinline__ v8i expand_f4(uint32_t w) {
+    return v8i{(int32_t)(w & 0x11111111u), (int32_t)((w >> 1) & 0x11111111u),
+               (int32_t)((w >> 2) & 0x11111111u), (int32_t)((w >> 3) & 0x11111111u), 0, 0, 0, 0};
+}
+
+template <int FFUN>
+__global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_Q];
+    int32_t ta, tb;
+    tile_of_f4(band_tile(g.tile_begin, g.n_tiles), ta, tb);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wa = wave >> 1, wb = wave & 1;
+    const int l16 = lane & 15, lg = lane >> 4;
+
+    /* Staging.  A K step holds 288 rows of 128 packed samples (one uint4
+     * each): A rows 0..191 (variant 64ta + r/3, plane r%3), then B rows.
+     * Thread t copies row t, and threads t < 32 also row 256 + t.           */
+    const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
+    const int64_t sstride = (int64_t)g.n_var_pad * 3; /* uint4 per K step */
+    const uint4 *src0 = tid < F4_QA ? base + (int64_t)ta * F4_QA + tid
+                                    : base + (int64_t)tb * (TB4 * 3) + (tid - F4_QA);
+    const bool has1 = tid < F4_Q - F4_THREADS;
+    const uint4 *src1 = base + (int64_t)tb * (TB4 * 3) + (F4_THREADS - F4_QA) + (has1 ? tid : 0);
+    const int last = g.n_ks - 1;
+
+    lds[0][tid] = src0[0];
+    if (has1) lds[0][F4_THREADS + tid] = src1[0];
+    uint4 pf0, pf1 = make_uint4(0u, 0u, 0u, 0u);
+    {
+        const int64_t o1 = (int64_t)(last < 1 ? last : 1) * sstride;
+        pf0 = src0[o1];
+        if (has1) pf1 = src1[o1];
+    }
+    __syncthreads();
+
+    /* lane's fragment words (dwords within a step buffer): A (i, p) is row
+     * (32wa + 16i + l16)*3 + p, B q is row 192 + (16wb + l16)*3 + q; the lane
+     * takes 32-sample word lg of the row.                                   */
+    const int offA = (32 * wa + l16) * 3 * 4 + lg;
+    const int offB = (F4_QA + (16 * wb + l16) * 3) * 4 + lg;
+    /* K steps [s_begin, s_end) into one class's accumulators; the staging
+     * pipeline runs on across the two calls.                                */
+    auto steps = [&](v4f (&acc)[2][3][3], int32_t s_begin, int32_t s_end) {
+        for (int32_t s = s_begin; s < s_end; s++) {
+            const uint32_t *L = reinterpret_cast<const uint32_t *>(lds[s & 1]);
+            uint32_t wA[2][3], wB[3];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int p = 0; p < 3; p++) wA[i][p] = L[offA + (48 * i + p) * 4];
+#pragma unroll
+            for (int q = 0; q < 3; q++) wB[q] = L[offB + q * 4];
+            /* stage step s+1 (a harmless extra copy on the last step: that
+             * buffer is not read again), prefetch step s+2 */
+            uint4 *W = lds[(s & 1) ^ 1];
+            W[tid] = pf0;
+            if (has1) W[F4_THREADS + tid] = pf1;
+            const int64_t on = (int64_t)(s + 2 <= last ? s + 2 : last) * sstride;
+            pf0 = src0[on];
+            if (has1) pf1 = src1[on];
+            v8i b[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) b[q] = expand_f4(wB[q]);
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int p = 0; p < 3; p++) {
+                    const v8i a = expand_f4(wA[i][p]);
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        acc[i][p][q] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                            a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, F4_SCALE, 0, F4_SCALE);
+                }
+            __syncthreads();
+        }
+    };
+    v4f acc0[2][3][3], acc1[2][3][3];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) acc0[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+    steps(acc0, 0, g.ks0);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) acc1[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+    steps(acc1, g.ks0, g.n_ks);
+
+    /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
+     * variant 32wa + 16i + 4lg + r against B variant 16wb + l16.           */
+    FilterConsts fc;
+    fc.thr_s = g.st->thr_score;
+    fc.kf = g.st->filt_k;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int32_t A = ta * TM + 32 * wa + 16 * i + 4 * lg + r;
+            const int32_t B = tb * TB4 + 16 * wb + l16;
+            if (A >= g.n_var || B > A) continue;
+            int32_t c[2][3][3];
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
+            filter_pair<FFUN, true>(g, fc, A, B, c);
         }
     }
 }

@@ -49,7 +49,8 @@ struct ScanState {
 
 struct CountArgs {
     const uint32_t *planes;   /* popcount layout [n_var_pad][3][wc] dwords          */
-    const uint64_t *planes_k; /* MFMA layout [n_ks][n_var_pad][3] 64-sample words   */
+    const uint64_t *planes_k; /* int8: [n_ks][n_var_pad][3] 64-sample words;       
+                               * fp4: [n_ks][n_var_pad*3] 128-sample rows          */
     int64_t tile_begin;       /* first linear tile of the band                      */
     int32_t n_tiles;          /* tiles in the band (= grid size)                    */
     int32_t n_var;
@@ -57,7 +58,7 @@ struct CountArgs {
     int32_t wc;               /* dwords per plane row (multiple of 32)              */
     int32_t w0;               /* class-0 dwords (multiple of 4)                     */
     int32_t wused;            /* dwords holding data (multiple of 4)                */
-    int32_t n_ks;             /* 64-sample K steps (wused / 2)                      */
+    int32_t n_ks;             /* K steps: 64 samples (int8), 128 samples (fp4)      */
     int32_t ks0;              /* K steps of class 0                                 */
     int32_t cls0_high;        /* MFMA: class 0 encoded as 0x80 (else class 1 is)    */
     int32_t pad;
@@ -109,6 +110,42 @@ struct FilterConsts {
     float kf;
 };
 
+/* Counts of the four row pairs (2A+ia, 2B+ib) of a variant pair, rp = 2ia+ib,
+ * for function f: with n = N11, a = NA, b = NB and v = V of one class,
+ *   f = 0 AND n, 1 NAND b-n, 2 ANDN a-n, 3 NANDN v-a-b+n, 4 XOR a+b-2n.      */
+__device__ __forceinline__ int32_t fun_count(int f, int32_t n, int32_t a, int32_t b, int32_t v) {
+    return f == 0 ? n : f == 1 ? b - n : f == 2 ? a - n : f == 3 ? v - a - b + n : a + b - 2 * n;
+}
+
+/* Appends the evaluations whose bit 5*rp+f is set in mask to the pending
+ * list.  Runs only for lanes with a survivor.  The row pair's operands
+ * advance through rotating registers: selecting among the counts with a
+ * runtime index would place them in scratch memory.                       */
+__device__ __forceinline__ void push_mask(const CountArgs &g, int32_t A, int32_t B,
+                                          const int32_t (&c)[2][3][3], uint32_t mask) {
+    const int32_t y = c[0][2][2], z = c[1][2][2];
+    int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
+    int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
+    int32_t a0 = c[0][0][2], a1 = c[1][0][2];
+    int32_t b0 = c[0][2][0], b1 = c[1][2][0], b0o = c[0][2][1], b1o = c[1][2][1];
+#pragma unroll 1
+    for (int rp = 0; rp < 4; rp++) {
+        uint32_t mb = (mask >> (5 * rp)) & 31u;
+        const uint64_t base = pack_idx(2 * (int64_t)A + (rp >> 1), 2 * (int64_t)B + (rp & 1), 0);
+        while (mb) {
+            const int f = __builtin_ctz(mb);
+            mb &= mb - 1;
+            push_pending(g, fun_count(f, n0a, a0, b0, y), fun_count(f, n1a, a1, b1, z), y, z,
+                         base | (uint64_t)f);
+        }
+        n0a = n0b; n0b = n0c; n0c = n0d;
+        n1a = n1b; n1b = n1c; n1c = n1d;
+        if (rp == 1) { a0 = c[0][1][2]; a1 = c[1][1][2]; }
+        int32_t t = b0; b0 = b0o; b0o = t;
+        t = b1; b1 = b1o; b1o = t;
+    }
+}
+
 /* Filters the 20 (or 12 on the diagonal) evaluations of variant pair (A, B)
  * from the nine plane products of each class, c[cls][p][q] =
  * popc(planeA[p] & planeB[q] & class cls) with p, q in {allele1, allele2,
@@ -116,72 +153,104 @@ struct FilterConsts {
  *
  * Entropy: -I <= thr needs I >= -thr =: It, and I_bits <= log2(1 + chi2/m)
  * (Jensen on KL), so a survivor must satisfy
- *     (wz - xy)^2 >= (2^It - 1) * y * z * xx * (m - xx),
- * evaluated in float with a 2^-16 relative margin (float error < 5e-7);
- * wz - xy == 0 is an independent table with I == 0.
- * Fisher: p >= P(observed table), so log10 P(obs) <= thr is required.      */
+ *     D^2 >= (2^It - 1) * y * z * xx * (m - xx),   D = wz - xy,
+ * tested as float(D)^2 > kf*y*z*(1 - 2^-16) * xx*(m - xx): the float error
+ * of both sides is below 2^-21 relative, so every table meeting the exact
+ * bound passes, and D == 0 (an independent table, I == 0) fails whenever
+ * kf > 0; kf == 0 admits everything.  D and xx are linear in the three
+ * per-row-pair quantities (N11, NA, NB): D_f = {Dn, Db-Dn, Da-Dn, Dn-Da-Db,
+ * Da+Db-2Dn} with Dq = q0*z - q1*y, and likewise xx_f from S = q0 + q1,
+ * computed with 24-bit products (all counts < 2^16).
+ * Fisher: p >= P(observed table), so log10 P(obs) <= thr is required.
+ *
+ * ROLLED: a runtime loop over the four row pairs keeps the code of callers
+ * with many pairs per thread small; its operands advance through rotating
+ * registers (an indexed array would be placed in scratch memory).          */
 template <int FFUN, bool ROLLED>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, const FilterConsts &fc, int32_t A,
                                             int32_t B, const int32_t (&c)[2][3][3]) {
     const int32_t y = c[0][2][2], z = c[1][2][2];
     const int32_t m = y + z;
-    const float kyz = fc.kf * (float)(y * z) * (1.0f - 1.0f / 65536.0f);
-    /* row pairs (2A+ia, 2B+ib); on the diagonal (2A, 2A+1) is not a pair.  */
-    auto row_pair = [&](int ia, int ib, int32_t n0, int32_t n1, int32_t a0, int32_t a1, int32_t b0,
-                        int32_t b1) {
-        const int32_t ws[5] = {n0, b0 - n0, a0 - n0, y - a0 - b0 + n0, a0 + b0 - 2 * n0};
-        const int32_t xs[5] = {n1, b1 - n1, a1 - n1, z - a1 - b1 + n1, a1 + b1 - 2 * n1};
-        const uint64_t base = pack_idx(2 * (int64_t)A + ia, 2 * (int64_t)B + ib, 0);
+    uint32_t mask = 0;
+    if (FFUN == GEPI_FFUNC_ENTROPY) {
+        const float mf = (float)m;
+        const float kyz = fc.kf * (float)__mul24(y, z) * (1.0f - 1.0f / 65536.0f);
+        const float bias = fc.kf == 0.0f ? -1.0f : 0.0f;
+        auto dq = [&](int32_t q0, int32_t q1) { return __mul24(q0, z) - __mul24(q1, y); };
+        auto sq = [&](int32_t q0, int32_t q1) { return (float)(q0 + q1); };
+        /* 5-bit pass pattern of one row pair */
+        auto row_bits = [&](int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db, float Sb) {
+            const int32_t Dab = Da + Db;
+            const float Sab = Sa + Sb;
+            const int32_t D[5] = {Dn, Db - Dn, Da - Dn, Dn - Dab, Dab - 2 * Dn};
+            const float X[5] = {Sn, Sb - Sn, Sa - Sn, (mf - Sab) + Sn, Sab - 2.0f * Sn};
+            uint32_t bits = 0;
 #pragma unroll
-        for (int f = 0; f < 5; f++) {
-            const int32_t w = ws[f], x = xs[f];
-            bool pass;
-            if (FFUN == GEPI_FFUNC_ENTROPY) {
-                const int32_t xx = w + x;
-                const int32_t q = xx * (m - xx);
-                const int32_t d = w * z - x * y;
-                const float fd = (float)d;
-                pass = (fc.kf == 0.0f) || (d != 0 && fd * fd >= kyz * (float)q);
-            } else {
-                pass = true;
-                if (fc.thr_s < 0.0) {
-                    const double *lf = g.lfact;
-                    double lp = ((lf[y] + lf[z]) + (lf[w + x] + lf[m - w - x])) -
-                                (((lf[w] + lf[y - w]) + (lf[x] + lf[z - x])) + lf[m]);
-                    pass = lp * 0.4342944819032518 - 1e-6 <= fc.thr_s;
-                }
+            for (int f = 0; f < 5; f++) {
+                const float q = X[f] * (mf - X[f]);
+                const float rhs = __builtin_fmaf(kyz, q, bias);
+                const float fd = (float)D[f];
+                bits |= (fd * fd > rhs) ? (1u << f) : 0u;
             }
-            if (pass) push_pending(g, w, x, y, z, base | (uint64_t)f);
-        }
-    };
-    if (ROLLED) {
-        /* A short runtime loop keeps the code of callers with many pairs per
-         * thread small.  The per-row-pair operands advance through rotating
-         * registers (no indexed array, which the compiler would place in
-         * scratch memory): row pair rp = 2*ia + ib.                        */
-        int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
-        int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
-        int32_t a0a = c[0][0][2], a0b = c[0][1][2], a1a = c[1][0][2], a1b = c[1][1][2];
-        int32_t b0a = c[0][2][0], b0b = c[0][2][1], b1a = c[1][2][0], b1b = c[1][2][1];
+            return bits;
+        };
+        const int32_t Da0 = dq(c[0][0][2], c[1][0][2]), Da1 = dq(c[0][1][2], c[1][1][2]);
+        const int32_t Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
+        const float Sa0 = sq(c[0][0][2], c[1][0][2]), Sa1 = sq(c[0][1][2], c[1][1][2]);
+        const float Sb0 = sq(c[0][2][0], c[1][2][0]), Sb1 = sq(c[0][2][1], c[1][2][1]);
+        if (ROLLED) {
+            int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
+            int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
+            int32_t Da = Da0, Db = Db0, Dbo = Db1;
+            float Sa = Sa0, Sb = Sb0, Sbo = Sb1;
 #pragma unroll 1
-        for (int rp = 0; rp < 4; rp++) {
-            const int ia = rp >> 1, ib = rp & 1;
-            if (!(A == B && ia < ib)) row_pair(ia, ib, n0a, n1a, a0a, a1a, b0a, b1a);
-            n0a = n0b; n0b = n0c; n0c = n0d;
-            n1a = n1b; n1b = n1c; n1c = n1d;
-            if (ib) { a0a = a0b; a1a = a1b; }
-            int32_t t = b0a; b0a = b0b; b0b = t;
-            t = b1a; b1a = b1b; b1b = t;
+            for (int rp = 0; rp < 4; rp++) {
+                mask |= row_bits(dq(n0a, n1a), sq(n0a, n1a), Da, Sa, Db, Sb) << (5 * rp);
+                n0a = n0b; n0b = n0c; n0c = n0d;
+                n1a = n1b; n1b = n1c; n1c = n1d;
+                if (rp == 1) { Da = Da1; Sa = Sa1; }
+                int32_t t = Db; Db = Dbo; Dbo = t;
+                float u = Sb; Sb = Sbo; Sbo = u;
+            }
+        } else {
+            mask = row_bits(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0) |
+                   row_bits(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) << 5 |
+                   row_bits(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) << 10 |
+                   row_bits(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) << 15;
         }
     } else {
+        if (fc.thr_s < 0.0) {
+            const double *lf = g.lfact;
+            auto rp_bits = [&](int32_t n0, int32_t n1, int32_t a0, int32_t a1, int32_t b0, int32_t b1) {
+                uint32_t bits = 0;
 #pragma unroll
-        for (int rp = 0; rp < 4; rp++) {
-            const int ia = rp >> 1, ib = rp & 1;
-            if (A == B && ia < ib) continue;
-            row_pair(ia, ib, c[0][ia][ib], c[1][ia][ib], c[0][ia][2], c[1][ia][2], c[0][2][ib],
-                     c[1][2][ib]);
+                for (int f = 0; f < 5; f++) {
+                    const int32_t w = fun_count(f, n0, a0, b0, y), x = fun_count(f, n1, a1, b1, z);
+                    const double lp = ((lf[y] + lf[z]) + (lf[w + x] + lf[m - w - x])) -
+                                      (((lf[w] + lf[y - w]) + (lf[x] + lf[z - x])) + lf[m]);
+                    bits |= (lp * 0.4342944819032518 - 1e-6 <= fc.thr_s) ? (1u << f) : 0u;
+                }
+                return bits;
+            };
+            int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
+            int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
+            int32_t a0 = c[0][0][2], a1 = c[1][0][2];
+            int32_t b0 = c[0][2][0], b1 = c[1][2][0], b0o = c[0][2][1], b1o = c[1][2][1];
+#pragma unroll 1
+            for (int rp = 0; rp < 4; rp++) {
+                mask |= rp_bits(n0a, n1a, a0, a1, b0, b1) << (5 * rp);
+                n0a = n0b; n0b = n0c; n0c = n0d;
+                n1a = n1b; n1b = n1c; n1c = n1d;
+                if (rp == 1) { a0 = c[0][1][2]; a1 = c[1][1][2]; }
+                int32_t t = b0; b0 = b0o; b0o = t;
+                t = b1; b1 = b1o; b1o = t;
+            }
+        } else {
+            mask = 0xFFFFFu;
         }
     }
+    if (A == B) mask &= ~(0x1Fu << 5); /* (2A, 2A+1) is not a row pair */
+    if (mask) push_mask(g, A, B, c, mask);
 }
 
 } // namespace gepi

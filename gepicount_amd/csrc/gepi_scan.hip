@@ -206,7 +206,7 @@ int validate_params(const gepi_params *par) {
     if (par->topmost < 0) return gepi_fail(GEPI_ERR_ARG, "topmost must be >= 0");
     if (par->ffun != GEPI_FFUNC_ENTROPY && par->ffun != GEPI_FFUNC_FISHERPV)
         return gepi_fail(GEPI_ERR_ARG, "unknown objective %d", par->ffun);
-    if (par->count_path < 0 || par->count_path > 2)
+    if (par->count_path < 0 || par->count_path > 3)
         return gepi_fail(GEPI_ERR_ARG, "unknown count path %d", par->count_path);
     return GEPI_OK;
 }
@@ -279,7 +279,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             cls0_high = big0 ? 1 : 0;
     }
     int path = par->count_path;
-    if (path == 0) path = cls0_high >= 0 ? 2 : 1;
+    if (path == 0) path = 3;
     if (path == 2 && cls0_high < 0)
         return gepi_fail(GEPI_ERR_LIMIT,
                          "MFMA count path needs a class with at most %d samples (have %d and %d)",
@@ -300,17 +300,20 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     std::vector<int32_t> perm((size_t)wc * 32, -1);
     for (int32_t i = 0; i < n0; i++) perm[i] = cls0[i];
     for (int32_t i = 0; i < n1; i++) perm[(size_t)w0 * 32 + i] = cls1[i];
-    const int32_t tv = path == 2 ? TM : TV;
     const int64_t n_var_pad = ((n_var + TM - 1) / TM) * TM;
+    /* workgroup tiles of the lower triangle: square tv x tv, or 64 x 32 (fp4) */
+    const int32_t tv = path == 2 ? TM : TV;
     const int64_t nt_side = n_var_pad / tv;
-    const int64_t n_tiles = nt_side * (nt_side + 1) / 2;
-    const int32_t n_ks = wused / 2;
+    const int64_t n_a4 = n_var_pad / TM;
+    const int64_t n_tiles = path == 3 ? n_a4 * (n_a4 + 1) : nt_side * (nt_side + 1) / 2;
+    const int64_t pairs_per_tile = path == 3 ? (int64_t)TM * TB4 : (int64_t)tv * tv;
+    const int32_t n_ks = path == 3 ? wused / 4 : wused / 2;
 
     if ((rc = ensure_tables(p, n0 + n1 + 1)) != GEPI_OK) return rc;
     if ((rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK) return rc;
-    if (path == 2 &&
+    if (path >= 2 &&
         (rc = grow(&p->d_clsk, &p->clsk_bytes,
-                   (size_t)std::max(n_ks, 1) * n_var_pad * 3 * 8)) != GEPI_OK)
+                   (size_t)std::max(n_ks, 1) * n_var_pad * 3 * (path == 3 ? 16 : 8))) != GEPI_OK)
         return rc;
     if ((rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK) return rc;
     unsigned long long cap = (unsigned long long)std::max<int64_t>(4 * k, 1 << 23);
@@ -346,6 +349,12 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
                                                        p->d_clsk);
             HIP_TRY(hipGetLastError());
         }
+        if (path == 3 && n_ks > 0) {
+            dim3 grid((unsigned)(n_var_pad * 3 / 64), (unsigned)((n_ks + 15) / 16));
+            k_transpose_f4<<<grid, 256, 0, p->stream>>>(p->d_cls, (int32_t)(n_var_pad * 3), wc, n_ks,
+                                                        reinterpret_cast<uint4 *>(p->d_clsk));
+            HIP_TRY(hipGetLastError());
+        }
     }
     HIP_TRY(hipEventRecord(p->ev1, p->stream));
 
@@ -372,7 +381,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ca.w0 = w0;
     ca.wused = wused;
     ca.n_ks = n_ks;
-    ca.ks0 = w0 / 2;
+    ca.ks0 = path == 3 ? w0 / 4 : w0 / 2;
     ca.cls0_high = cls0_high > 0 ? 1 : 0;
     ca.st = p->d_state;
     ca.pend = p->d_pend;
@@ -386,7 +395,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ea.lhi = p->tabs.lhi;
     ea.llo = p->tabs.llo;
 
-    const int64_t evals_per_tile = (int64_t)tv * tv * 20;
+    const int64_t evals_per_tile = pairs_per_tile * 20;
     int64_t max_band = std::max<int64_t>(64, (int64_t)(2.0e9 / (double)evals_per_tile));
     if (par->max_grid_area > 0)
         max_band = std::max<int64_t>(1, ((int64_t)par->max_grid_area << 20) / evals_per_tile);
@@ -404,7 +413,12 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         k_reset_pending<<<1, 1, 0, p->stream>>>(p->d_state);
         HIP_TRY(hipEventRecord(p->ev0, p->stream));
         const unsigned grid = (unsigned)(t1 - t0);
-        if (path == 2) {
+        if (path == 3) {
+            if (entropy)
+                k_count_f4<GEPI_FFUNC_ENTROPY><<<grid, F4_THREADS, 0, p->stream>>>(ca);
+            else
+                k_count_f4<GEPI_FFUNC_FISHERPV><<<grid, F4_THREADS, 0, p->stream>>>(ca);
+        } else if (path == 2) {
             if (entropy)
                 k_count_mfma<GEPI_FFUNC_ENTROPY><<<grid, MF_THREADS, 0, p->stream>>>(ca);
             else

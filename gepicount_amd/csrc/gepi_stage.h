@@ -11,6 +11,8 @@
  *                  popcount layout [n_var_pad][3][wc] dwords.
  * k_transpose_k    popcount layout -> MFMA layout [n_ks][n_var_pad][3] of
  *                  64-sample words, so that one K step of a tile is contiguous.
+ * k_transpose_f4   popcount layout -> fp4 MFMA layout [n_k4][n_var_pad*3] of
+ *                  128-sample rows (uint4), one K step of a tile contiguous.
  */
 #ifndef GEPI_STAGE_H
 #define GEPI_STAGE_H
@@ -94,6 +96,27 @@ __global__ void __launch_bounds__(256) k_transpose_k(const uint32_t *__restrict_
         const int32_t k = idx / 96, vp = idx - 96 * k;
         const int32_t s = s0 + k;
         if (s < n_ks) out[((int64_t)s * n_var_pad + v0) * 3 + vp] = tile[vp][k];
+    }
+}
+
+/* rows = n_var_pad*3 (a multiple of 64); grid (rows / 64, ceil(n_k4 / 16)). */
+__global__ void __launch_bounds__(256) k_transpose_f4(const uint32_t *__restrict__ cls,
+                                                       int32_t rows, int32_t wc, int32_t n_k4,
+                                                       uint4 *__restrict__ out) {
+    __shared__ uint4 tile[64][17];
+    const int32_t r0 = blockIdx.x * 64, s0 = blockIdx.y * 16;
+    for (int32_t idx = threadIdx.x; idx < 64 * 16; idx += blockDim.x) {
+        const int32_t r = idx >> 4, k = idx & 15;
+        const int32_t s = s0 + k;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (s < n_k4) val = *reinterpret_cast<const uint4 *>(cls + (int64_t)(r0 + r) * wc + 4 * s);
+        tile[r][k] = val;
+    }
+    __syncthreads();
+    for (int32_t idx = threadIdx.x; idx < 64 * 16; idx += blockDim.x) {
+        const int32_t k = idx >> 6, r = idx & 63;
+        const int32_t s = s0 + k;
+        if (s < n_k4) out[(int64_t)s * rows + r0 + r] = tile[r][k];
     }
 }
 

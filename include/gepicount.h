@@ -70,7 +70,8 @@ typedef struct gepi_params {
     int32_t gpu_num;       /* -G: HIP device ordinal                           */
     int32_t ffun;          /* -f: gepi_ffunc                                   */
     int64_t topmost;       /* -N: results to keep                              */
-    int32_t count_path;    /* 0 = fastest available, 1 = popcount, 2 = MFMA    */
+    int32_t count_path;    /* 0 = fastest (fp4 MFMA), 1 = popcount (VALU),     
+                            * 2 = int8 MFMA, 3 = fp4 (e2m1) MFMA                */
     int32_t intermediate;  /* -D: caller prints intermediate results (CLI)     */
     int32_t blk_x, blk_y;  /* --block-sizes: accepted for compatibility        */
     int32_t max_grid_area; /* --max-grid-area: cap on evals per launch, in Mi  */

@@ -39,7 +39,7 @@ def random_panel(seed, n_var, n, miss, cls_p=(0.5, 0.45, 0.05)):
     return bits, valid, m0, m1
 
 
-PATHS = [1, 2]  # 1 = popcount (VALU), 2 = int8 MFMA
+PATHS = [1, 2, 3]  # 1 = popcount (VALU), 2 = int8 MFMA, 3 = fp4 MFMA
 
 
 @pytest.mark.parametrize("path", PATHS)
@@ -175,10 +175,11 @@ def test_overflow_recovery_path(monkeypatch, path):
 
 
 def test_mfma_class_encoding_limits():
-    """The MFMA path keeps one class in the low 14 bits of its accumulators
-    (at most 16383 samples there).  A class above that puts the other class
-    in the low bits; when both exceed it, the automatic path is popcount and
-    an explicit MFMA request fails with GEPI_ERR_LIMIT."""
+    """The int8 MFMA path keeps one class in the low 14 bits of its
+    accumulators (at most 16383 samples there).  A class above that puts the
+    other class in the low bits; when both exceed it, an explicit int8 request
+    fails with GEPI_ERR_LIMIT, while the fp4 path (separate accumulators, the
+    automatic choice) and the popcount path still count exactly."""
     rng = np.random.default_rng(71)
     n = 16600
     codes = orc.random_codes(rng, 3, n, 0.01)
@@ -193,10 +194,12 @@ def test_mfma_class_encoding_limits():
     bits, valid = orc.planes_from_codes(codes)
     cls = np.where(np.arange(n) < 16500, 0, 1)      # both classes > 16383
     m0, m1 = orc.masks_from_classes(cls)
-    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=10**6)
-    assert st["count_path"] == 1
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), 10**6)
-    assert_same(got, ref, "popcount-fallback")
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=10**6)
+    assert st["count_path"] == 3
+    assert_same(got, ref, "fp4-two-big-classes")
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=10**6, count_path=1)
+    assert_same(got, ref, "popcount-two-big-classes")
     with pytest.raises(ge.GepiError) as e:
         ge.process_gpu_1(bits, valid, m0, m1, n, topmost=10, count_path=2)
     assert e.value.code == -3
