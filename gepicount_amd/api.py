@@ -51,7 +51,8 @@ class ScanStats(ctypes.Structure):
                 ("ms_prepare", ctypes.c_double), ("ms_select", ctypes.c_double),
                 ("evals", ctypes.c_int64), ("exact_evals", ctypes.c_int64),
                 ("bands", ctypes.c_int32), ("count_path", ctypes.c_int32),
-                ("n_class0", ctypes.c_int32), ("n_class1", ctypes.c_int32)]
+                ("n_class0", ctypes.c_int32), ("n_class1", ctypes.c_int32),
+                ("pend_overflows", ctypes.c_int32), ("cand_overflows", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -72,6 +72,8 @@ template <int FFUN>
 __global__ void __launch_bounds__(POPC_THREADS) k_count_popc(CountArgs g) {
     __shared__ __attribute__((aligned(16))) uint32_t sA[TV * VSTRIDE];
     __shared__ __attribute__((aligned(16))) uint32_t sB[TV * VSTRIDE];
+    __shared__ PendStage stage;
+    stage_init(stage);
     int32_t ta, tb;
     tile_of(band_tile(g.tile_begin, g.n_tiles), ta, tb);
 
@@ -132,9 +134,10 @@ __global__ void __launch_bounds__(POPC_THREADS) k_count_popc(CountArgs g) {
                 for (int p = 0; p < 3; p++)
 #pragma unroll
                     for (int q = 0; q < 3; q++) c[k][p][q] = (int32_t)acc[k][i][j][p][q];
-            filter_pair<FFUN, false>(g, fc, A, B, c);
+            filter_pair<FFUN, false>(g, stage, fc, A, B, c);
         }
     }
+    stage_flush(g, stage);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -160,6 +163,8 @@ __device__ __forceinline__ void expand_half(uint8_t *__restrict__ dst, uint32_t 
 template <int FFUN>
 __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
     __shared__ __attribute__((aligned(16))) uint8_t lds[2][MF_ROWS * MF_ROW];
+    __shared__ PendStage stage;
+    stage_init(stage);
     int32_t ta, tb;
     tile_of(band_tile(g.tile_begin, g.n_tiles), ta, tb);
 
@@ -283,10 +288,11 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
                         c[0][p][q] = c0_high ? hi : lo;
                         c[1][p][q] = c0_high ? lo : hi;
                     }
-                filter_pair<FFUN, true>(g, fc, A, B, c);
+                filter_pair<FFUN, true>(g, stage, fc, A, B, c);
             }
         }
     }
+    stage_flush(g, stage);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -325,6 +331,8 @@ __device__ __forceinline__ v8i expand_f4(uint32_t w) {
 template <int FFUN>
 __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_Q];
+    __shared__ PendStage stage;
+    stage_init(stage);
     int32_t ta, tb;
     tile_of_f4(band_tile(g.tile_begin, g.n_tiles), ta, tb);
 
@@ -431,9 +439,10 @@ __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
 #pragma unroll
                     for (int q = 0; q < 3; q++)
                         c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
-            filter_pair<FFUN, true>(g, fc, A, B, c);
+            filter_pair<FFUN, true>(g, stage, fc, A, B, c);
         }
     }
+    stage_flush(g, stage);
 }
 
 } // namespace gepi

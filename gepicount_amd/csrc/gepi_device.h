@@ -17,6 +17,7 @@ constexpr int TV = 32;              /* popcount kernel: variants per tile side  
 constexpr int TM = 64;              /* MFMA kernel: variants per tile side           */
 constexpr int SEL_SMALL = 1024;     /* radix select finishes in LDS below this size  */
 constexpr int MFMA_LOW_MAX = 16383; /* class count limit of the 0x01-encoded class   */
+constexpr int64_t TILE_EVALS_MAX = (int64_t)TM * TM * 20; /* evaluations of the largest tile */
 
 /* A candidate for the top-k: exact score plus counts.  32 bytes. */
 struct Cand {
@@ -78,6 +79,18 @@ __host__ __device__ inline void tile_of(int64_t t, int32_t &ta, int32_t &tb) {
     tb = (int32_t)(t - a * (a + 1) / 2);
 }
 
+/* Linear workgroup tile of variant pair (a >= b).  Square tiles of side tv
+ * (popcount, int8 MFMA): ta*(ta+1)/2 + tb.  fp4 tiles of 64 x 32 variants
+ * (rect != 0): ta*(ta+1) + tb with ta = a/64, tb = b/32.                   */
+__host__ __device__ inline int64_t tile_index(int64_t a, int64_t b, int32_t tv, int32_t rect) {
+    if (rect) {
+        const int64_t ta = a / 64, tb = b / 32;
+        return ta * (ta + 1) + tb;
+    }
+    const int64_t ta = a / tv, tb = b / tv;
+    return ta * (ta + 1) / 2 + tb;
+}
+
 /* Workgroup -> tile of the band.  Consecutive workgroups are dispatched to
  * the 8 XCDs round robin; each XCD gets a contiguous run of tiles so that
  * tiles sharing a row panel share that XCD's L2.                            */
@@ -88,19 +101,61 @@ __device__ __forceinline__ int64_t band_tile(int64_t tile_begin, int32_t n_tiles
     return tile_begin + (int64_t)xcd * per + (xcd < rem ? xcd : rem) + slot;
 }
 
-__device__ __forceinline__ void push_pending(const CountArgs &g, int32_t w, int32_t x, int32_t y,
-                                             int32_t z, uint64_t idx) {
+/* Survivors of one workgroup are staged in LDS (LDS atomics) and written to
+ * the pending list with one global reservation per workgroup; past
+ * PEND_LDS they go to the pending list directly.                          */
+constexpr int PEND_LDS = 256;
+struct PendStage {
+    Pend rec[PEND_LDS];
+    uint32_t n;
+    uint32_t pad;
+    unsigned long long base;
+};
+
+__device__ __forceinline__ void push_global(const CountArgs &g, const Pend &p) {
     unsigned long long pos = atomicAdd(&g.st->pend_count, 1ull);
-    if (pos < g.pcap) {
-        Pend p;
-        p.w = (uint16_t)w;
-        p.x = (uint16_t)x;
-        p.y = (uint16_t)y;
-        p.z = (uint16_t)z;
-        p.idx = idx;
+    if (pos < g.pcap)
         g.pend[pos] = p;
-    } else {
+    else
         atomicOr(&g.st->pend_overflow, 1u);
+}
+
+__device__ __forceinline__ void push_pending(const CountArgs &g, PendStage &ps, int32_t w,
+                                             int32_t x, int32_t y, int32_t z, uint64_t idx) {
+    Pend p;
+    p.w = (uint16_t)w;
+    p.x = (uint16_t)x;
+    p.y = (uint16_t)y;
+    p.z = (uint16_t)z;
+    p.idx = idx;
+    const uint32_t pos = atomicAdd(&ps.n, 1u);
+    if (pos < (uint32_t)PEND_LDS)
+        ps.rec[pos] = p;
+    else
+        push_global(g, p);
+}
+
+/* Called by every thread of the workgroup at kernel start (before the first
+ * barrier) and after the last push, respectively.                          */
+__device__ __forceinline__ void stage_init(PendStage &ps) {
+    if (threadIdx.x == 0) ps.n = 0;
+}
+
+__device__ __forceinline__ void stage_flush(const CountArgs &g, PendStage &ps) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t n = ps.n < (uint32_t)PEND_LDS ? ps.n : (uint32_t)PEND_LDS;
+        ps.n = n;
+        ps.base = n ? atomicAdd(&g.st->pend_count, (unsigned long long)n) : 0ull;
+    }
+    __syncthreads();
+    const uint32_t n = ps.n;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long pos = ps.base + i;
+        if (pos < g.pcap)
+            g.pend[pos] = ps.rec[i];
+        else
+            atomicOr(&g.st->pend_overflow, 1u);
     }
 }
 
@@ -121,7 +176,7 @@ __device__ __forceinline__ int32_t fun_count(int f, int32_t n, int32_t a, int32_
  * list.  Runs only for lanes with a survivor.  The row pair's operands
  * advance through rotating registers: selecting among the counts with a
  * runtime index would place them in scratch memory.                       */
-__device__ __forceinline__ void push_mask(const CountArgs &g, int32_t A, int32_t B,
+__device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, int32_t A, int32_t B,
                                           const int32_t (&c)[2][3][3], uint32_t mask) {
     const int32_t y = c[0][2][2], z = c[1][2][2];
     int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
@@ -135,7 +190,7 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, int32_t A, int32_t
         while (mb) {
             const int f = __builtin_ctz(mb);
             mb &= mb - 1;
-            push_pending(g, fun_count(f, n0a, a0, b0, y), fun_count(f, n1a, a1, b1, z), y, z,
+            push_pending(g, ps, fun_count(f, n0a, a0, b0, y), fun_count(f, n1a, a1, b1, z), y, z,
                          base | (uint64_t)f);
         }
         n0a = n0b; n0b = n0c; n0c = n0d;
@@ -149,7 +204,7 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, int32_t A, int32_t
 /* Filters the 20 (or 12 on the diagonal) evaluations of variant pair (A, B)
  * from the nine plane products of each class, c[cls][p][q] =
  * popc(planeA[p] & planeB[q] & class cls) with p, q in {allele1, allele2,
- * valid}.  Survivors are appended to the pending list.
+ * valid}.  Survivors are staged for the pending list.
  *
  * Entropy: -I <= thr needs I >= -thr =: It, and I_bits <= log2(1 + chi2/m)
  * (Jensen on KL), so a survivor must satisfy
@@ -167,8 +222,8 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, int32_t A, int32_t
  * with many pairs per thread small; its operands advance through rotating
  * registers (an indexed array would be placed in scratch memory).          */
 template <int FFUN, bool ROLLED>
-__device__ __forceinline__ void filter_pair(const CountArgs &g, const FilterConsts &fc, int32_t A,
-                                            int32_t B, const int32_t (&c)[2][3][3]) {
+__device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
+                                            int32_t A, int32_t B, const int32_t (&c)[2][3][3]) {
     const int32_t y = c[0][2][2], z = c[1][2][2];
     const int32_t m = y + z;
     uint32_t mask = 0;
@@ -250,7 +305,7 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, const FilterCons
         }
     }
     if (A == B) mask &= ~(0x1Fu << 5); /* (2A, 2A+1) is not a row pair */
-    if (mask) push_mask(g, A, B, c, mask);
+    if (mask) push_mask(g, ps, A, B, c, mask);
 }
 
 } // namespace gepi

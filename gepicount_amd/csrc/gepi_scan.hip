@@ -95,7 +95,8 @@ struct gepi_panel {
     size_t perm_bytes = 0;
     Cand *d_cand[2] = {nullptr, nullptr};
     Pend *d_pend = nullptr;
-    unsigned long long buf_cap = 0;
+    unsigned long long buf_cap = 0;  /* records in each candidate buffer */
+    unsigned long long pend_cap = 0; /* records in the pending list       */
     ScanState *d_state = nullptr;
     uint64_t *d_small = nullptr;
     DeviceTables tabs;
@@ -316,23 +317,39 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
                    (size_t)std::max(n_ks, 1) * n_var_pad * 3 * (path == 3 ? 16 : 8))) != GEPI_OK)
         return rc;
     if ((rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK) return rc;
-    unsigned long long cap = (unsigned long long)std::max<int64_t>(4 * k, 1 << 23);
+    /* Candidate buffers: between bands at most k records are kept, and a
+     * one-tile band admits at most TILE_EVALS_MAX, so cap >= k +
+     * TILE_EVALS_MAX lets a band shrunk to one tile always complete.        */
+    const unsigned long long cap_min = (unsigned long long)k + TILE_EVALS_MAX + 1024;
+    unsigned long long cap = std::max<unsigned long long>(
+        cap_min, (unsigned long long)std::max<int64_t>(4 * k, 1 << 23));
     if (const char *env = getenv("GEPI_CAND_CAP")) {
         /* capacity override for exercising the overflow recovery in tests */
-        cap = (unsigned long long)std::max<long long>(atoll(env), 2 * k + 1024);
-        cap = std::max<unsigned long long>(cap, (unsigned long long)TM * TM * 20 + 1024);
+        cap = std::max<unsigned long long>((unsigned long long)std::max(atoll(env), 0ll), cap_min);
+    }
+    unsigned long long pcap = cap;
+    if (const char *env = getenv("GEPI_PEND_CAP")) {
+        /* pending-list capacity override for tests: a list larger than the
+         * candidate buffers lets one band admit more candidates than fit,
+         * which exercises the candidate overflow recovery                 */
+        pcap = std::max<unsigned long long>((unsigned long long)std::max(atoll(env), 0ll),
+                                            (unsigned long long)TILE_EVALS_MAX + 1024);
     }
     if (cap > p->buf_cap) {
         hipFree(p->d_cand[0]);
         hipFree(p->d_cand[1]);
-        hipFree(p->d_pend);
         p->d_cand[0] = p->d_cand[1] = nullptr;
-        p->d_pend = nullptr;
         p->buf_cap = 0;
         HIP_TRY(hipMalloc(&p->d_cand[0], sizeof(Cand) * cap));
         HIP_TRY(hipMalloc(&p->d_cand[1], sizeof(Cand) * cap));
-        HIP_TRY(hipMalloc(&p->d_pend, sizeof(Pend) * cap));
         p->buf_cap = cap;
+    }
+    if (pcap > p->pend_cap) {
+        hipFree(p->d_pend);
+        p->d_pend = nullptr;
+        p->pend_cap = 0;
+        HIP_TRY(hipMalloc(&p->d_pend, sizeof(Pend) * pcap));
+        p->pend_cap = pcap;
     }
 
     HIP_TRY(hipMemcpyAsync(p->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice,
@@ -385,12 +402,12 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ca.cls0_high = cls0_high > 0 ? 1 : 0;
     ca.st = p->d_state;
     ca.pend = p->d_pend;
-    ca.pcap = cap;
+    ca.pcap = pcap;
     ca.lfact = p->tabs.lfact;
     ExactArgs ea;
     ea.st = p->d_state;
     ea.pend = p->d_pend;
-    ea.pcap = cap;
+    ea.pcap = pcap;
     ea.cap = cap;
     ea.lhi = p->tabs.lhi;
     ea.llo = p->tabs.llo;
@@ -400,6 +417,13 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     if (par->max_grid_area > 0)
         max_band = std::max<int64_t>(1, ((int64_t)par->max_grid_area << 20) / evals_per_tile);
     int64_t band = std::max<int64_t>(1, std::min<int64_t>(max_band, (2 * k) / evals_per_tile + 1));
+    /* band-size override for exercising the overflow recovery in tests: the
+     * first band and the growth limit, ignoring the survivor-rate estimate */
+    int64_t forced_band = 0;
+    if (const char *env = getenv("GEPI_BAND_TILES")) {
+        forced_band = std::max<long long>(1, atoll(env));
+        band = forced_band;
+    }
     double ms_count = 0.0, ms_top = 0.0;
     int64_t t0 = 0;
     unsigned long long count = 0, exact_total = 0;
@@ -446,6 +470,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         if (hs.pend_overflow) {
             /* the pending list lost survivors; the candidates are untouched
              * (k_exact skipped), so redo the band in smaller pieces        */
+            st_local.pend_overflows++;
             band = std::max<int64_t>(1, (t1 - t0) / 4);
             ms_top += ms_since(ts);
             continue;
@@ -455,13 +480,14 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         if (hs.overflow) {
             /* candidates lost: threshold from what was kept, drop this band's
              * records and redo the band                                     */
+            st_local.cand_overflows++;
             const unsigned long long kept = std::min<unsigned long long>(hs.count, cap);
             hs.count = kept;
             hs.overflow = 0;
             HIP_TRY(hipMemcpyAsync(p->d_state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
             launch_select(p, p->d_cand[cur], (uint32_t)k, kept);
             k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
-                                                   t0, t1, tv);
+                                                   t0, t1, tv, path == 3 ? 1 : 0);
             k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
             HIP_TRY(hipGetLastError());
             cur ^= 1;
@@ -476,7 +502,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         if (hs.count > (unsigned long long)k) {
             launch_select(p, p->d_cand[cur], (uint32_t)k, hs.count);
             k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
-                                                   0, 0, tv);
+                                                   0, 0, tv, path == 3 ? 1 : 0);
             k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
             HIP_TRY(hipGetLastError());
             cur ^= 1;
@@ -491,6 +517,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         const int64_t room = (int64_t)(cap - (unsigned long long)k) / 2;
         const int64_t next = rate > 0.0 ? (int64_t)((double)room / rate) : max_band;
         band = std::max<int64_t>(1, std::min<int64_t>({max_band, next, band * 4}));
+        if (forced_band) band = std::min<int64_t>(forced_band, (t1 - t0) * 4);
         t0 = t1;
     }
     /* final extraction: exactly k records, sorted by key on the host */

@@ -185,11 +185,11 @@ __global__ void __launch_bounds__(1024) k_sel_small(ScanState *st, const uint64_
     }
 }
 
-/* Keeps records with key <= the selected key; records of tiles in
- * [drop_t0, drop_t1) (tile side tv) are dropped as well.                    */
+/* Keeps records with key <= the selected key; records of workgroup tiles in
+ * [drop_t0, drop_t1) (geometry tv, rect as in tile_index) are dropped too.  */
 __global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, ScanState *st,
                                                   Cand *__restrict__ dst, int64_t drop_t0,
-                                                  int64_t drop_t1, int32_t tv) {
+                                                  int64_t drop_t1, int32_t tv, int32_t rect) {
     const uint64_t khi = st->key_hi, klo = st->key_lo;
     const unsigned long long n = st->count;
     for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -198,10 +198,9 @@ __global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, S
         uint64_t hi = order_bits(c.score);
         bool keep = hi < khi || (hi == khi && c.idx <= klo);
         if (keep && drop_t1 > drop_t0) {
-            int64_t a = (int64_t)(c.idx >> 32) >> 1;
-            int64_t b = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull) >> 1;
-            int64_t ta = a / tv, tb = b / tv;
-            int64_t t = ta * (ta + 1) / 2 + tb;
+            const int64_t a = (int64_t)(c.idx >> 32) >> 1;
+            const int64_t b = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull) >> 1;
+            const int64_t t = tile_index(a, b, tv, rect);
             keep = t < drop_t0 || t >= drop_t1;
         }
         if (keep) {

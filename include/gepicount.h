@@ -89,6 +89,8 @@ typedef struct gepi_scan_stats {
     int32_t bands;           /* count-kernel launches                          */
     int32_t count_path;      /* path actually used                             */
     int32_t n_class0, n_class1;
+    int32_t pend_overflows;  /* bands redone: filter survivors overflowed      */
+    int32_t cand_overflows;  /* bands redone: admitted candidates overflowed   */
 } gepi_scan_stats;
 
 /* Defaults of src/gEpiCount.cpp:34-39 and :100-115. */

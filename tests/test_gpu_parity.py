@@ -164,14 +164,38 @@ def test_overlapping_and_empty_masks():
 
 
 @pytest.mark.parametrize("path", PATHS)
-def test_overflow_recovery_path(monkeypatch, path):
-    """A small pending/candidate buffer forces bands to overflow and be redone."""
+def test_pending_overflow_recovery(monkeypatch, path):
+    """A forced 32-tile first band with no threshold yet overflows the pending
+    list; the band is redone in smaller pieces."""
     bits, valid, m0, m1 = random_panel(51, 500, 300, 0.01)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 300), 100)
     monkeypatch.setenv("GEPI_CAND_CAP", "1")
+    monkeypatch.setenv("GEPI_BAND_TILES", "32")
     got, st = ge.process_gpu_1(bits, valid, m0, m1, 300, topmost=100, count_path=path)
     monkeypatch.delenv("GEPI_CAND_CAP")
-    assert_same(got, ref, "overflow")
+    monkeypatch.delenv("GEPI_BAND_TILES")
+    assert st["pend_overflows"] > 0
+    assert_same(got, ref, "pending-overflow")
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_candidate_overflow_recovery(monkeypatch, path):
+    """Forced 8-tile bands with a pending list larger than the candidate
+    buffers admit more candidates than fit: the threshold is taken from the
+    records kept, the band's records are dropped by workgroup tile (square or
+    64 x 32 geometry) and the band is redone.  Duplicated or lost records
+    would break the exact match."""
+    bits, valid, m0, m1 = random_panel(53, 300, 200, 0.01)
+    k = 20000
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 200), k)
+    monkeypatch.setenv("GEPI_CAND_CAP", "1")
+    monkeypatch.setenv("GEPI_PEND_CAP", "4000000")
+    monkeypatch.setenv("GEPI_BAND_TILES", "8")
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, 200, topmost=k, count_path=path)
+    for v in ("GEPI_CAND_CAP", "GEPI_PEND_CAP", "GEPI_BAND_TILES"):
+        monkeypatch.delenv(v)
+    assert st["cand_overflows"] > 0
+    assert_same(got, ref, "candidate-overflow")
 
 
 def test_mfma_class_encoding_limits():
