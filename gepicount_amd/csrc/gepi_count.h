@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
 #pragma unroll
                     for (int q = 0; q < 3; q++)
                         c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
-            filter_pair<FFUN, true>(g, stage, fc, A, B, c);
+            filter_pair<FFUN, false>(g, stage, fc, A, B, c);
         }
     }
     stage_flush(g, stage);
