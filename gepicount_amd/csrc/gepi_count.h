@@ -117,9 +117,7 @@ __global__ void __launch_bounds__(POPC_THREADS) k_count_popc(CountArgs g) {
         for (int kk = split; kk < end; kk += 4) popc_step<1>(sA, sB, ty, tx, kk, acc);
     }
 
-    FilterConsts fc;
-    fc.thr_s = g.st->thr_score;
-    fc.kf = g.st->filt_k;
+    const FilterConsts fc = filter_consts(g.st);
 #pragma unroll
     for (int i = 0; i < 2; i++) {
 #pragma unroll
@@ -265,9 +263,7 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
 
     /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
      * variant 32wa + 16i + 4lg + r against B variant 32wb + 16j + l16.     */
-    FilterConsts fc;
-    fc.thr_s = g.st->thr_score;
-    fc.kf = g.st->filt_k;
+    const FilterConsts fc = filter_consts(g.st);
     const bool c0_high = g.cls0_high != 0; /* class 0 held in the high part */
 #pragma unroll
     for (int i = 0; i < 2; i++) {
@@ -421,9 +417,7 @@ __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
 
     /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
      * variant 32wa + 16i + 4lg + r against B variant 16wb + l16.           */
-    FilterConsts fc;
-    fc.thr_s = g.st->thr_score;
-    fc.kf = g.st->filt_k;
+    const FilterConsts fc = filter_consts(g.st);
 #pragma unroll
     for (int i = 0; i < 2; i++) {
 #pragma unroll

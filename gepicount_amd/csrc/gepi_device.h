@@ -159,11 +159,23 @@ __device__ __forceinline__ void stage_flush(const CountArgs &g, PendStage &ps) {
     }
 }
 
-/* Per-scan filter constants, read once per workgroup. */
+/* Per-scan filter constants, read once per workgroup (filter_consts). */
 struct FilterConsts {
-    double thr_s;
-    float kf;
+    double thr_s; /* admission score threshold                                 */
+    float kf;     /* entropy: chi-square coefficient, 0 = admit everything      */
+    float a_thr;  /* Fisher: -(thr + 1e-6) * ln 10, less a float error margin   */
 };
+
+__device__ __forceinline__ FilterConsts filter_consts(const ScanState *st) {
+    FilterConsts fc;
+    fc.thr_s = st->thr_score;
+    fc.kf = st->filt_k;
+    const double a = -(fc.thr_s + 1e-6) * 2.302585092994046;
+    /* float rounding of a and of the per-pair log below stay under 1e-6
+     * relative; the margin keeps the derived bound conservative            */
+    fc.a_thr = fc.thr_s < 0.0 ? (float)(a - (fabs(a) + 10.0) * 1e-6) : 0.0f;
+    return fc;
+}
 
 /* Counts of the four row pairs (2A+ia, 2B+ib) of a variant pair, rp = 2ia+ib,
  * for function f: with n = N11, a = NA, b = NB and v = V of one class,
@@ -175,10 +187,14 @@ __device__ __forceinline__ int32_t fun_count(int f, int32_t n, int32_t a, int32_
 /* Appends the evaluations whose bit 5*rp+f is set in mask to the pending
  * list.  Runs only for lanes with a survivor.  The row pair's operands
  * advance through rotating registers: selecting among the counts with a
- * runtime index would place them in scratch memory.                       */
-__device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, int32_t A, int32_t B,
-                                          const int32_t (&c)[2][3][3], uint32_t mask) {
-    const int32_t y = c[0][2][2], z = c[1][2][2];
+ * runtime index would place them in scratch memory.  Fisher: the survivors
+ * of the chi-square bound are checked against log10 P(obs) <= thr (p >=
+ * P(observed table)) with the log-factorial table first.                 */
+template <int FFUN>
+__device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
+                                          int32_t A, int32_t B, const int32_t (&c)[2][3][3],
+                                          uint32_t mask) {
+    const int32_t y = c[0][2][2], z = c[1][2][2], m = y + z;
     int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
     int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
     int32_t a0 = c[0][0][2], a1 = c[1][0][2];
@@ -190,8 +206,14 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, int
         while (mb) {
             const int f = __builtin_ctz(mb);
             mb &= mb - 1;
-            push_pending(g, ps, fun_count(f, n0a, a0, b0, y), fun_count(f, n1a, a1, b1, z), y, z,
-                         base | (uint64_t)f);
+            const int32_t w = fun_count(f, n0a, a0, b0, y), x = fun_count(f, n1a, a1, b1, z);
+            if (FFUN == GEPI_FFUNC_FISHERPV && fc.thr_s < 0.0) {
+                const double *lf = g.lfact;
+                const double lp = ((lf[y] + lf[z]) + (lf[w + x] + lf[m - w - x])) -
+                                  (((lf[w] + lf[y - w]) + (lf[x] + lf[z - x])) + lf[m]);
+                if (!(lp * 0.4342944819032518 - 1e-6 <= fc.thr_s)) continue;
+            }
+            push_pending(g, ps, w, x, y, z, base | (uint64_t)f);
         }
         n0a = n0b; n0b = n0c; n0c = n0d;
         n1a = n1b; n1b = n1c; n1c = n1d;
@@ -206,17 +228,24 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, int
  * popc(planeA[p] & planeB[q] & class cls) with p, q in {allele1, allele2,
  * valid}.  Survivors are staged for the pending list.
  *
- * Entropy: -I <= thr needs I >= -thr =: It, and I_bits <= log2(1 + chi2/m)
- * (Jensen on KL), so a survivor must satisfy
- *     D^2 >= (2^It - 1) * y * z * xx * (m - xx),   D = wz - xy,
- * tested as float(D)^2 > kf*y*z*(1 - 2^-16) * xx*(m - xx): the float error
- * of both sides is below 2^-21 relative, so every table meeting the exact
- * bound passes, and D == 0 (an independent table, I == 0) fails whenever
- * kf > 0; kf == 0 admits everything.  D and xx are linear in the three
- * per-row-pair quantities (N11, NA, NB): D_f = {Dn, Db-Dn, Da-Dn, Dn-Da-Db,
- * Da+Db-2Dn} with Dq = q0*z - q1*y, and likewise xx_f from S = q0 + q1,
- * computed with 24-bit products (all counts < 2^16).
- * Fisher: p >= P(observed table), so log10 P(obs) <= thr is required.
+ * Both objectives reduce to a chi-square test.  With I the mutual
+ * information of the class x f table (nats) and chi2/m = D^2/(y z q),
+ * D = wz - xy, q = xx(m - xx), Jensen on the KL divergence gives
+ * I <= ln(1 + chi2/m), so I >= T requires D^2 >= expm1(T) * y * z * q.
+ * Entropy: -I_bits <= thr needs T = -thr * ln 2 (kf = 2^-thr - 1, made on
+ * the host side of the threshold update, rounded down).
+ * Fisher: p >= P(obs) = C(y,w) C(z,x) / C(m,xx), and e^{nH(k/n)} / sqrt(2n)
+ * <= C(n,k) <= e^{nH(k/n)} (H in nats) give P(obs) >= e^{-mI} / (2 sqrt(yz)),
+ * so log10 P(obs) - 1e-6 <= thr needs
+ *     T = (-(thr + 1e-6) ln 10 - ln(4yz) / 2) / m   (y, z clamped to >= 1);
+ * its survivors are then checked exactly against the log-factorial table.
+ * The test is float(D)^2 > K*y*z*(1 - 2^-16) * q: the float error of both
+ * sides is below 2^-21 relative, so every table meeting the exact bound
+ * passes, and D == 0 (an independent table, I == 0) fails whenever K > 0;
+ * K == 0 admits everything.  D and xx are linear in the three per-row-pair
+ * quantities (N11, NA, NB): D_f = {Dn, Db-Dn, Da-Dn, Dn-Da-Db, Da+Db-2Dn}
+ * with Dq = q0*z - q1*y, and likewise xx_f from S = q0 + q1, computed with
+ * 24-bit products (all counts < 2^16).
  *
  * ROLLED: a runtime loop over the four row pairs keeps the code of callers
  * with many pairs per thread small; its operands advance through rotating
@@ -226,86 +255,63 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
                                             int32_t A, int32_t B, const int32_t (&c)[2][3][3]) {
     const int32_t y = c[0][2][2], z = c[1][2][2];
     const int32_t m = y + z;
-    uint32_t mask = 0;
-    if (FFUN == GEPI_FFUNC_ENTROPY) {
-        const float mf = (float)m;
-        const float kyz = fc.kf * (float)__mul24(y, z) * (1.0f - 1.0f / 65536.0f);
-        const float bias = fc.kf == 0.0f ? -1.0f : 0.0f;
-        auto dq = [&](int32_t q0, int32_t q1) { return __mul24(q0, z) - __mul24(q1, y); };
-        auto sq = [&](int32_t q0, int32_t q1) { return (float)(q0 + q1); };
-        /* 5-bit pass pattern of one row pair */
-        auto row_bits = [&](int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db, float Sb) {
-            const int32_t Dab = Da + Db;
-            const float Sab = Sa + Sb;
-            const int32_t D[5] = {Dn, Db - Dn, Da - Dn, Dn - Dab, Dab - 2 * Dn};
-            const float X[5] = {Sn, Sb - Sn, Sa - Sn, (mf - Sab) + Sn, Sab - 2.0f * Sn};
-            uint32_t bits = 0;
-#pragma unroll
-            for (int f = 0; f < 5; f++) {
-                const float q = X[f] * (mf - X[f]);
-                const float rhs = __builtin_fmaf(kyz, q, bias);
-                const float fd = (float)D[f];
-                bits |= (fd * fd > rhs) ? (1u << f) : 0u;
-            }
-            return bits;
-        };
-        const int32_t Da0 = dq(c[0][0][2], c[1][0][2]), Da1 = dq(c[0][1][2], c[1][1][2]);
-        const int32_t Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
-        const float Sa0 = sq(c[0][0][2], c[1][0][2]), Sa1 = sq(c[0][1][2], c[1][1][2]);
-        const float Sb0 = sq(c[0][2][0], c[1][2][0]), Sb1 = sq(c[0][2][1], c[1][2][1]);
-        if (ROLLED) {
-            int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
-            int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
-            int32_t Da = Da0, Db = Db0, Dbo = Db1;
-            float Sa = Sa0, Sb = Sb0, Sbo = Sb1;
-#pragma unroll 1
-            for (int rp = 0; rp < 4; rp++) {
-                mask |= row_bits(dq(n0a, n1a), sq(n0a, n1a), Da, Sa, Db, Sb) << (5 * rp);
-                n0a = n0b; n0b = n0c; n0c = n0d;
-                n1a = n1b; n1b = n1c; n1c = n1d;
-                if (rp == 1) { Da = Da1; Sa = Sa1; }
-                int32_t t = Db; Db = Dbo; Dbo = t;
-                float u = Sb; Sb = Sbo; Sbo = u;
-            }
-        } else {
-            mask = row_bits(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0) |
-                   row_bits(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) << 5 |
-                   row_bits(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) << 10 |
-                   row_bits(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) << 15;
-        }
-    } else {
+    const float mf = (float)m;
+    float K = fc.kf;
+    if (FFUN == GEPI_FFUNC_FISHERPV) {
+        K = 0.0f;
         if (fc.thr_s < 0.0) {
-            const double *lf = g.lfact;
-            auto rp_bits = [&](int32_t n0, int32_t n1, int32_t a0, int32_t a1, int32_t b0, int32_t b1) {
-                uint32_t bits = 0;
-#pragma unroll
-                for (int f = 0; f < 5; f++) {
-                    const int32_t w = fun_count(f, n0, a0, b0, y), x = fun_count(f, n1, a1, b1, z);
-                    const double lp = ((lf[y] + lf[z]) + (lf[w + x] + lf[m - w - x])) -
-                                      (((lf[w] + lf[y - w]) + (lf[x] + lf[z - x])) + lf[m]);
-                    bits |= (lp * 0.4342944819032518 - 1e-6 <= fc.thr_s) ? (1u << f) : 0u;
-                }
-                return bits;
-            };
-            int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
-            int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
-            int32_t a0 = c[0][0][2], a1 = c[1][0][2];
-            int32_t b0 = c[0][2][0], b1 = c[1][2][0], b0o = c[0][2][1], b1o = c[1][2][1];
-#pragma unroll 1
-            for (int rp = 0; rp < 4; rp++) {
-                mask |= rp_bits(n0a, n1a, a0, a1, b0, b1) << (5 * rp);
-                n0a = n0b; n0b = n0c; n0c = n0d;
-                n1a = n1b; n1b = n1c; n1c = n1d;
-                if (rp == 1) { a0 = c[0][1][2]; a1 = c[1][1][2]; }
-                int32_t t = b0; b0 = b0o; b0o = t;
-                t = b1; b1 = b1o; b1o = t;
-            }
-        } else {
-            mask = 0xFFFFFu;
+            const float yz1 = (float)(y > 1 ? y : 1) * (float)(z > 1 ? z : 1);
+            const float T = (fc.a_thr - 0.5f * __logf(4.0f * yz1)) / (m > 1 ? mf : 1.0f);
+            if (T > 0.0f) K = expm1f(T) * (1.0f - 1e-5f);
         }
     }
+    const float kyz = K * (float)__mul24(y, z) * (1.0f - 1.0f / 65536.0f);
+    const float bias = K == 0.0f ? -1.0f : 0.0f;
+    auto dq = [&](int32_t q0, int32_t q1) { return __mul24(q0, z) - __mul24(q1, y); };
+    auto sq = [&](int32_t q0, int32_t q1) { return (float)(q0 + q1); };
+    /* 5-bit pass pattern of one row pair */
+    auto row_bits = [&](int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db, float Sb) {
+        const int32_t Dab = Da + Db;
+        const float Sab = Sa + Sb;
+        const int32_t D[5] = {Dn, Db - Dn, Da - Dn, Dn - Dab, Dab - 2 * Dn};
+        const float X[5] = {Sn, Sb - Sn, Sa - Sn, (mf - Sab) + Sn, Sab - 2.0f * Sn};
+        uint32_t bits = 0;
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            const float q = X[f] * (mf - X[f]);
+            const float rhs = __builtin_fmaf(kyz, q, bias);
+            const float fd = (float)D[f];
+            bits |= (fd * fd > rhs) ? (1u << f) : 0u;
+        }
+        return bits;
+    };
+    const int32_t Da0 = dq(c[0][0][2], c[1][0][2]), Da1 = dq(c[0][1][2], c[1][1][2]);
+    const int32_t Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
+    const float Sa0 = sq(c[0][0][2], c[1][0][2]), Sa1 = sq(c[0][1][2], c[1][1][2]);
+    const float Sb0 = sq(c[0][2][0], c[1][2][0]), Sb1 = sq(c[0][2][1], c[1][2][1]);
+    uint32_t mask = 0;
+    if (ROLLED) {
+        int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
+        int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
+        int32_t Da = Da0, Db = Db0, Dbo = Db1;
+        float Sa = Sa0, Sb = Sb0, Sbo = Sb1;
+#pragma unroll 1
+        for (int rp = 0; rp < 4; rp++) {
+            mask |= row_bits(dq(n0a, n1a), sq(n0a, n1a), Da, Sa, Db, Sb) << (5 * rp);
+            n0a = n0b; n0b = n0c; n0c = n0d;
+            n1a = n1b; n1b = n1c; n1c = n1d;
+            if (rp == 1) { Da = Da1; Sa = Sa1; }
+            int32_t t = Db; Db = Dbo; Dbo = t;
+            float u = Sb; Sb = Sbo; Sbo = u;
+        }
+    } else {
+        mask = row_bits(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0) |
+               row_bits(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) << 5 |
+               row_bits(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) << 10 |
+               row_bits(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) << 15;
+    }
     if (A == B) mask &= ~(0x1Fu << 5); /* (2A, 2A+1) is not a row pair */
-    if (mask) push_mask(g, ps, A, B, c, mask);
+    if (mask) push_mask<FFUN>(g, ps, fc, A, B, c, mask);
 }
 
 } // namespace gepi
