@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--topk", type=int, default=10000)
     ap.add_argument("--ffun", type=int, default=0)
     ap.add_argument("--count-path", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+    ap.add_argument("--cpu-seconds", type=float, default=25.0,
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter scale)")
