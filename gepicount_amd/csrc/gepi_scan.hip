@@ -93,6 +93,8 @@ struct gepi_panel {
     size_t clsk_bytes = 0;
     int32_t *d_perm = nullptr;
     size_t perm_bytes = 0;
+    CompressWord *d_cw = nullptr; /* fp4 path: class compression words */
+    size_t cw_bytes = 0;
     Cand *d_cand[2] = {nullptr, nullptr};
     Pend *d_pend = nullptr;
     unsigned long long buf_cap = 0;  /* records in each candidate buffer */
@@ -176,6 +178,7 @@ void panel_free(gepi_panel *p) {
     hipFree(p->d_cls);
     hipFree(p->d_clsk);
     hipFree(p->d_perm);
+    hipFree(p->d_cw);
     hipFree(p->d_cand[0]);
     hipFree(p->d_cand[1]);
     hipFree(p->d_pend);
@@ -311,12 +314,34 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const int32_t n_ks = path == 3 ? wused / 4 : wused / 2;
 
     if ((rc = ensure_tables(p, n0 + n1 + 1)) != GEPI_OK) return rc;
-    if ((rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK) return rc;
+    if (path != 3 &&
+        (rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK)
+        return rc;
     if (path >= 2 &&
         (rc = grow(&p->d_clsk, &p->clsk_bytes,
                    (size_t)std::max(n_ks, 1) * n_var_pad * 3 * (path == 3 ? 16 : 8))) != GEPI_OK)
         return rc;
     if ((rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK) return rc;
+    /* fp4 path: per 32-sample word of the panel, the compression of each
+     * class (mask bits at or beyond n_samples are dropped) */
+    std::vector<CompressWord> cw;
+    if (path == 3) {
+        const int32_t nw32 = 2 * p->w64;
+        cw.resize((size_t)2 * std::max(nw32, 1));
+        for (int c = 0; c < 2; c++) {
+            const uint64_t *mk = c == 0 ? mask_ctrl : mask_case;
+            for (int32_t j = 0; j < nw32; j++) {
+                uint32_t m = 0;
+                if (mk) m = (uint32_t)(mk[j >> 1] >> (32 * (j & 1)));
+                const int32_t lo = 32 * j;
+                if (lo >= p->n_samples) m = 0;
+                else if (p->n_samples - lo < 32) m &= (1u << (p->n_samples - lo)) - 1u;
+                cw[(size_t)c * nw32 + j] = make_compress_word(m);
+            }
+        }
+        if ((rc = grow(&p->d_cw, &p->cw_bytes, cw.size() * sizeof(CompressWord))) != GEPI_OK)
+            return rc;
+    }
     /* Candidate buffers: between bands at most k records are kept, and a
      * one-tile band admits at most TILE_EVALS_MAX, so cap >= k +
      * TILE_EVALS_MAX lets a band shrunk to one tile always complete.        */
@@ -352,10 +377,22 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         p->pend_cap = pcap;
     }
 
-    HIP_TRY(hipMemcpyAsync(p->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice,
-                           p->stream));
+    if (path == 3)
+        HIP_TRY(hipMemcpyAsync(p->d_cw, cw.data(), cw.size() * sizeof(CompressWord),
+                               hipMemcpyHostToDevice, p->stream));
+    else
+        HIP_TRY(hipMemcpyAsync(p->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice,
+                               p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
-    {
+    if (path == 3) {
+        const int64_t rows = n_var_pad * 3;
+        if (n_ks > 0) {
+            k_class_f4<<<(unsigned)((rows + 255) / 256), 256, 0, p->stream>>>(
+                p->d_nat, (int32_t)n_var, p->w64, p->d_cw, rows, w0 / 4,
+                reinterpret_cast<uint4 *>(p->d_clsk));
+            HIP_TRY(hipGetLastError());
+        }
+    } else {
         size_t lds = (size_t)3 * (p->w64 > 0 ? p->w64 : 1) * 8;
         k_gather_class<<<(unsigned)n_var_pad, 256, lds, p->stream>>>(
             p->d_nat, (int32_t)n_var, p->w64, p->d_perm, wc, wused, p->d_cls);
@@ -366,12 +403,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
                                                        p->d_clsk);
             HIP_TRY(hipGetLastError());
         }
-        if (path == 3 && n_ks > 0) {
-            dim3 grid((unsigned)(n_var_pad * 3 / 64), (unsigned)((n_ks + 15) / 16));
-            k_transpose_f4<<<grid, 256, 0, p->stream>>>(p->d_cls, (int32_t)(n_var_pad * 3), wc, n_ks,
-                                                        reinterpret_cast<uint4 *>(p->d_clsk));
-            HIP_TRY(hipGetLastError());
-        }
+
     }
     HIP_TRY(hipEventRecord(p->ev1, p->stream));
 

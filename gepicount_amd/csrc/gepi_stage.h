@@ -13,6 +13,10 @@
  *                  64-sample words, so that one K step of a tile is contiguous.
  * k_transpose_f4   popcount layout -> fp4 MFMA layout [n_k4][n_var_pad*3] of
  *                  128-sample rows (uint4), one K step of a tile contiguous.
+ * k_class_f4       panel -> fp4 MFMA layout directly: per plane row, the
+ *                  samples of each class are compressed (stable) out of the
+ *                  file order with mask-driven bit compression, and streamed
+ *                  out in 128-sample rows (class 0 steps, then class 1).
  */
 #ifndef GEPI_STAGE_H
 #define GEPI_STAGE_H
@@ -118,6 +122,140 @@ __global__ void __launch_bounds__(256) k_transpose_f4(const uint32_t *__restrict
         const int32_t s = s0 + k;
         if (s < n_k4) out[(int64_t)s * rows + r0 + r] = tile[r][k];
     }
+}
+
+/* Compression of one 32-bit sample word for one class: x & m moved to the
+ * low popc(m) bits, order kept, by five conditional shifts whose move masks
+ * depend on m only (Hacker's Delight, "compress").  Built on the host once
+ * per scan; the same for every plane row.                                   */
+struct CompressWord {
+    uint32_t m;     /* class mask word                                      */
+    uint32_t mv[5]; /* move masks of the stages shifting by 1, 2, 4, 8, 16 */
+    uint32_t cnt;   /* popc(m)                                              */
+    uint32_t pad;
+};
+
+__host__ __device__ inline CompressWord make_compress_word(uint32_t m) {
+    CompressWord c;
+    c.m = m;
+    c.cnt = 0;
+    for (uint32_t t = m; t; t &= t - 1) c.cnt++;
+    uint32_t mk = ~m << 1; /* zeros to the right of each position */
+    for (int i = 0; i < 5; i++) {
+        uint32_t mp = mk ^ (mk << 1); /* parallel suffix (xor) */
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        const uint32_t mv = mp & m;
+        c.mv[i] = mv;
+        m = (m ^ mv) | (mv >> (1u << i));
+        mk &= ~mp;
+    }
+    c.pad = 0;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t compress_word(uint32_t x, const CompressWord &c) {
+    x &= c.m;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const uint32_t t = x & c.mv[i];
+        x = (x ^ t) | (t >> (1u << i));
+    }
+    return x;
+}
+
+/* A stream of bits appended in order and written as 128-bit rows
+ * out[step * rows + r].  Every thread of a launch appends the same counts in
+ * the same order, so the bookkeeping is uniform.                            */
+struct BitStream {
+    uint64_t acc;
+    int32_t na, nq;
+    int64_t step;
+    uint32_t q0, q1, q2, q3;
+
+    __device__ __forceinline__ void init(int64_t s0) {
+        acc = 0;
+        na = nq = 0;
+        step = s0;
+        q0 = q1 = q2 = q3 = 0;
+    }
+    __device__ __forceinline__ void emit(uint32_t w, uint4 *out, int64_t rows, int64_t r) {
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = w;
+        if (++nq == 4) {
+            if (r < rows) out[step * rows + r] = make_uint4(q0, q1, q2, q3);
+            step++;
+            nq = 0;
+        }
+    }
+    __device__ __forceinline__ void append(uint32_t bits, int32_t k, uint4 *out, int64_t rows,
+                                           int64_t r) {
+        acc |= (uint64_t)bits << na;
+        na += k;
+        if (na >= 32) {
+            emit((uint32_t)acc, out, rows, r);
+            acc >>= 32;
+            na -= 32;
+        }
+    }
+    /* zero-pads the last 128-bit row of the stream */
+    __device__ __forceinline__ void finish(uint4 *out, int64_t rows, int64_t r) {
+        if (na > 0) {
+            emit((uint32_t)acc, out, rows, r);
+            acc = 0;
+            na = 0;
+        }
+        while (nq != 0) emit(0u, out, rows, r);
+    }
+};
+
+/* One thread per plane row r = v*3 + p of [0, rows) (rows = n_var_pad * 3;
+ * rows of padding variants v >= n_var are zero).  cw: [2][2*w64]
+ * CompressWord, class 0 then class 1.  Class 0 fills K steps [0, ks0),
+ * class 1 steps [ks0, ks0 + ceil(n1 / 128)).  The panel rows of a
+ * workgroup are staged through LDS 8 words at a time (coalesced reads).     */
+constexpr int CLS_CHUNK = 8;
+__global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
+                                                   int32_t w64, const CompressWord *__restrict__ cw,
+                                                   int64_t rows, int32_t ks0,
+                                                   uint4 *__restrict__ out) {
+    __shared__ uint64_t srow[256][CLS_CHUNK + 1];
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int64_t r = r0 + threadIdx.x;
+    const int64_t real_rows = (int64_t)n_var * 3;
+    BitStream s0, s1;
+    s0.init(0);
+    s1.init(ks0);
+    const CompressWord *c0 = cw, *c1 = cw + 2 * w64;
+    for (int32_t base = 0; base < w64; base += CLS_CHUNK) {
+        __syncthreads();
+        for (int32_t idx = threadIdx.x; idx < 256 * CLS_CHUNK; idx += 256) {
+            const int32_t rl = idx / CLS_CHUNK, wd = idx % CLS_CHUNK;
+            const int64_t rr = r0 + rl;
+            uint64_t val = 0;
+            if (rr < real_rows && base + wd < w64) val = nat[rr * w64 + base + wd];
+            srow[rl][wd] = val;
+        }
+        __syncthreads();
+        const int32_t nw = w64 - base < CLS_CHUNK ? w64 - base : CLS_CHUNK;
+        for (int32_t wd = 0; wd < nw; wd++) {
+            const uint64_t x = srow[threadIdx.x][wd];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint32_t xh = (uint32_t)(x >> (32 * h));
+                const int32_t j = 2 * (base + wd) + h;
+                const CompressWord a = c0[j], b = c1[j];
+                if (a.cnt) s0.append(compress_word(xh, a), (int32_t)a.cnt, out, rows, r);
+                if (b.cnt) s1.append(compress_word(xh, b), (int32_t)b.cnt, out, rows, r);
+            }
+        }
+    }
+    s0.finish(out, rows, r);
+    s1.finish(out, rows, r);
 }
 
 } // namespace gepi
