@@ -29,7 +29,8 @@ RESULT_DTYPE = np.dtype([("sA", "<i8"), ("sB", "<i8"), ("w", "<i4"), ("x", "<i4"
 EXPORTS = ("gepi_params_init", "gepi_process_gpu_1", "gepi_panel_create", "gepi_panel_create_bed",
            "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_plink_open",
            "gepi_plink_close", "gepi_plink_check_quality", "gepi_print_interactions",
-           "gepi_fisher_pval", "gepi_last_error", "gepi_version", "gepi_device_count")
+           "gepi_fisher_pval", "gepi_last_error", "gepi_version", "gepi_device_count",
+           "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch")
 
 
 class GepiError(RuntimeError):
@@ -53,6 +54,22 @@ class ScanStats(ctypes.Structure):
                 ("bands", ctypes.c_int32), ("count_path", ctypes.c_int32),
                 ("n_class0", ctypes.c_int32), ("n_class1", ctypes.c_int32),
                 ("pend_overflows", ctypes.c_int32), ("cand_overflows", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("panel", ctypes.c_void_p), ("mask_ctrl", ctypes.POINTER(ctypes.c_uint64)),
+                ("mask_case", ctypes.POINTER(ctypes.c_uint64)), ("topmost", ctypes.c_int64),
+                ("ffun", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("tops", ctypes.c_void_p), ("n_tops", ctypes.c_int64), ("stats", ScanStats)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("ms_batch", ctypes.c_double), ("ms_prepare", ctypes.c_double),
+                ("ms_count_kernels", ctypes.c_double), ("evals", ctypes.c_int64),
+                ("rounds", ctypes.c_int32), ("items_failed", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -115,9 +132,14 @@ def lib():
     l.gepi_last_error.restype = ctypes.c_char_p
     l.gepi_version.restype = ctypes.c_char_p
     l.gepi_device_count.argtypes = [ctypes.POINTER(ctypes.c_int32)]
+    l.gepi_engine_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp)]
+    l.gepi_engine_destroy.argtypes = [vp]
+    l.gepi_engine_scan_batch.argtypes = [vp, ctypes.POINTER(BatchItem), ctypes.c_int32,
+                                         ctypes.POINTER(BatchStats)]
     for name in ("gepi_process_gpu_1", "gepi_panel_create", "gepi_panel_create_bed",
                  "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_plink_open",
-                 "gepi_plink_check_quality", "gepi_print_interactions", "gepi_device_count"):
+                 "gepi_plink_check_quality", "gepi_print_interactions", "gepi_device_count",
+                 "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch"):
         getattr(l, name).restype = ctypes.c_int
     _lib = l
     return l
@@ -232,6 +254,60 @@ class Panel:
     def close(self):
         if self._h:
             _check(lib().gepi_panel_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """Batched scans on one device (gepi_engine_*): many scans advance together
+    in rounds of three launches.  Each item's result equals Panel.scan's."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().gepi_engine_create(int(device), ctypes.byref(h)))
+        self._h = h
+
+    def scan_batch(self, items):
+        """items: sequence of dicts {panel, mask_ctrl, mask_case, topmost, ffun}.
+        Returns (list of (results, stats, status), batch stats dict); a failed
+        item has status < 0 and no results."""
+        n = len(items)
+        arr = (BatchItem * max(n, 1))()
+        keep = []
+        for i, it in enumerate(items):
+            panel = it["panel"]
+            m0, m1 = _masks(it.get("mask_ctrl"), it.get("mask_case"), panel.words)
+            topmost = int(it.get("topmost", 1000000))
+            out = _result_buffer(topmost, panel.n_rows)
+            keep.append((m0, m1, out))
+            arr[i].panel = panel._h
+            arr[i].mask_ctrl = _u64p(m0)
+            arr[i].mask_case = _u64p(m1)
+            arr[i].topmost = topmost
+            arr[i].ffun = int(it.get("ffun", GEPI_FFUNC_ENTROPY))
+            arr[i].tops = out.ctypes.data
+        bs = BatchStats()
+        _check(lib().gepi_engine_scan_batch(self._h, arr, n, ctypes.byref(bs)))
+        res = []
+        for i in range(n):
+            out = keep[i][2]
+            res.append((out[:arr[i].n_tops].copy(), arr[i].stats.as_dict(), int(arr[i].status)))
+        return res, bs.as_dict()
+
+    def close(self):
+        if self._h:
+            _check(lib().gepi_engine_destroy(self._h))
             self._h = None
 
     def __enter__(self):

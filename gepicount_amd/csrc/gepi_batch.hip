@@ -1,0 +1,499 @@
+/*
+ * gepi_batch.hip -- the batched scan engine (Engine) and its C ABI
+ * (gepi_engine_*).  Kernels are in gepi_batch.h; the per-scan band logic is
+ * the one of run_scan (gepi_scan.hip) applied to every active scan at once.
+ * No counterpart in the reference, which runs one scan per process
+ * (src/gEpiCount.cpp:84-87).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/gepicount.h"
+#include "gepi_batch.h"
+#include "gepi_engine.h"
+#include "gepi_host.h"
+
+using namespace gepi;
+
+namespace gepi_host {
+
+namespace {
+
+constexpr int64_t F4_TILE_EVALS = (int64_t)TM * TB4 * 20;
+constexpr int EXACT_BLOCKS_X = 64; /* blocks per scan of the batched exact pass */
+
+double ms_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
+size_t round_block(size_t bytes) {
+    /* 64 KiB granules below 64 MiB, 1 MiB above: cached blocks are reused for
+     * any request between half and all of their size                       */
+    const size_t g = bytes < ((size_t)64 << 20) ? ((size_t)64 << 10) : ((size_t)1 << 20);
+    return std::max<size_t>(g, (bytes + g - 1) / g * g);
+}
+
+unsigned long long env_ull(const char *name, unsigned long long dflt) {
+    const char *v = getenv(name);
+    if (!v) return dflt;
+    const long long x = atoll(v);
+    return x > 0 ? (unsigned long long)x : dflt;
+}
+
+} // namespace
+
+int Engine::init(int dev) {
+    device = dev;
+    int rc;
+    if ((rc = set_device(dev)) != GEPI_OK) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ev0_));
+    HIP_TRY(hipEventCreate(&ev1_));
+    return GEPI_OK;
+}
+
+void Engine::release() {
+    hipSetDevice(device);
+    if (stream) hipStreamSynchronize(stream);
+    for (ScanItem *it : active_) drop(it, GEPI_ERR_INTERNAL);
+    active_.clear();
+    for (auto &b : free_) hipFree(b.second);
+    free_.clear();
+    for (auto &b : live_) hipFree(b.first);
+    live_.clear();
+    for (int i = 0; i < 2; i++) hipFree(d_bdesc_[i]);
+    hipFree(d_edesc_);
+    hipFree(d_tdesc_);
+    hipFree(d_rep_);
+    hipFree(tabs_.lhi);
+    hipFree(tabs_.llo);
+    hipFree(tabs_.lfact);
+    if (ev0_) hipEventDestroy(ev0_);
+    if (ev1_) hipEventDestroy(ev1_);
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+void *Engine::dalloc(size_t bytes) {
+    const size_t sz = round_block(bytes);
+    void *p = nullptr;
+    auto it = free_.lower_bound(sz);
+    if (it != free_.end() && it->first <= 2 * sz) {
+        p = it->second;
+        live_[p] = it->first;
+        in_use_ += it->first;
+        free_.erase(it);
+        return p;
+    }
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        /* return the cache to the device and try once more */
+        for (auto &b : free_) hipFree(b.second);
+        free_.clear();
+        if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
+    }
+    live_[p] = sz;
+    in_use_ += sz;
+    return p;
+}
+
+void Engine::dfree(void *p) {
+    if (!p) return;
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    in_use_ -= it->second;
+    free_.insert({it->second, p});
+    live_.erase(it);
+}
+
+int Engine::grow_desc(size_t n) {
+    if (n <= desc_cap_) return GEPI_OK;
+    const size_t c = std::max<size_t>(n, 2 * desc_cap_);
+    for (int i = 0; i < 2; i++) {
+        hipFree(d_bdesc_[i]);
+        d_bdesc_[i] = nullptr;
+    }
+    hipFree(d_edesc_);
+    hipFree(d_tdesc_);
+    hipFree(d_rep_);
+    d_edesc_ = nullptr;
+    d_tdesc_ = nullptr;
+    d_rep_ = nullptr;
+    desc_cap_ = 0;
+    for (int i = 0; i < 2; i++) HIP_TRY(hipMalloc(&d_bdesc_[i], sizeof(BatchDesc) * c));
+    HIP_TRY(hipMalloc(&d_edesc_, sizeof(ExactDesc) * c));
+    HIP_TRY(hipMalloc(&d_tdesc_, sizeof(TopkDesc) * c));
+    HIP_TRY(hipMalloc(&d_rep_, sizeof(ItemReport) * c));
+    desc_cap_ = c;
+    return GEPI_OK;
+}
+
+size_t Engine::estimate_footprint(const gepi_panel *p, int64_t topmost) {
+    const int64_t n_var_pad = ((p->n_var + TM - 1) / TM) * TM;
+    const int64_t n_k4 = (p->n_samples + 127) / 128 + 2;
+    const int64_t R = 2 * p->n_var;
+    const int64_t k = std::min<int64_t>(topmost, 5 * (R * (R + 1) / 2));
+    const unsigned long long cap =
+        std::max<unsigned long long>((unsigned long long)k + TILE_EVALS_MAX + 1024, 4ull * k);
+    return round_block((size_t)n_k4 * n_var_pad * 3 * 16) + round_block(cap * sizeof(Cand)) +
+           round_block(cap * sizeof(Pend)) + round_block((size_t)4 * p->w64 * sizeof(CompressWord)) +
+           round_block(sizeof(ScanState));
+}
+
+void Engine::drop(ScanItem *it, int status) {
+    dfree(it->d_layout);
+    dfree(it->d_cw);
+    dfree(it->d_pend);
+    dfree(it->d_cand);
+    dfree(it->d_state);
+    it->d_layout = nullptr;
+    it->d_cw = nullptr;
+    it->d_pend = nullptr;
+    it->d_cand = nullptr;
+    it->d_state = nullptr;
+    it->footprint = 0;
+    it->status = status;
+    it->done = true;
+    it->stats.ms_scan = ms_since(it->t_added);
+}
+
+int Engine::add(ScanItem *it) {
+    it->t_added = std::chrono::steady_clock::now();
+    it->done = false;
+    it->status = GEPI_OK;
+    it->results.clear();
+    memset(&it->stats, 0, sizeof it->stats);
+    gepi_panel *p = it->panel;
+    if (!p) {
+        it->status = gepi_fail(GEPI_ERR_ARG, "null panel");
+        it->done = true;
+        return it->status;
+    }
+    if (p->device != device || it->topmost < 0 ||
+        (it->ffun != GEPI_FFUNC_ENTROPY && it->ffun != GEPI_FFUNC_FISHERPV)) {
+        it->status = gepi_fail(GEPI_ERR_ARG, "scan item: panel on device %d (engine %d), "
+                                             "topmost %lld, objective %d",
+                               p->device, device, (long long)it->topmost, it->ffun);
+        it->done = true;
+        return it->status;
+    }
+    build_compress_words(it->mask_ctrl, it->mask_case, p->n_samples, p->w64, it->h_cw, it->n0,
+                         it->n1);
+    it->n_var = p->n_var;
+    const int64_t R = 2 * it->n_var;
+    const int64_t total = 5 * (R * (R + 1) / 2);
+    it->stats.evals = total;
+    it->stats.count_path = 3;
+    it->stats.n_class0 = it->n0;
+    it->stats.n_class1 = it->n1;
+    if ((int64_t)it->n0 + it->n1 > GEPI_MAX_SAMPLES) {
+        it->status = gepi_fail(GEPI_ERR_LIMIT, "%d class samples exceed the limit of %d",
+                               it->n0 + it->n1, GEPI_MAX_SAMPLES);
+        it->done = true;
+        return it->status;
+    }
+    it->k = std::min<int64_t>(it->topmost, total);
+    if (it->k > (int64_t)(1u << 30)) {
+        it->status = gepi_fail(GEPI_ERR_LIMIT, "topmost too large");
+        it->done = true;
+        return it->status;
+    }
+    if (it->k == 0) {
+        it->done = true;
+        it->stats.ms_scan = ms_since(it->t_added);
+        return GEPI_OK;
+    }
+    int rc;
+    if ((rc = ensure_tables(tabs_, it->n0 + it->n1 + 1)) != GEPI_OK) {
+        it->status = rc;
+        it->done = true;
+        return rc;
+    }
+    const int32_t w0 = ((it->n0 + 127) / 128) * 4, w1 = ((it->n1 + 127) / 128) * 4;
+    it->n_k4 = (w0 + w1) / 4;
+    it->ks0 = w0 / 4;
+    it->n_var_pad = ((it->n_var + TM - 1) / TM) * TM;
+    const int64_t n_a4 = it->n_var_pad / TM;
+    it->n_tiles = n_a4 * (n_a4 + 1);
+    const unsigned long long cap_min = (unsigned long long)it->k + TILE_EVALS_MAX + 1024;
+    it->cap = std::max<unsigned long long>(cap_min, 4ull * (unsigned long long)it->k);
+    if (getenv("GEPI_CAND_CAP")) it->cap = std::max(env_ull("GEPI_CAND_CAP", 0), cap_min);
+    it->pcap = it->cap;
+    if (getenv("GEPI_PEND_CAP"))
+        it->pcap = std::max<unsigned long long>(env_ull("GEPI_PEND_CAP", 0),
+                                                (unsigned long long)TILE_EVALS_MAX + 1024);
+    const size_t lay = (size_t)std::max(it->n_k4, 1) * it->n_var_pad * 3 * sizeof(uint4);
+    it->d_layout = (uint4 *)dalloc(lay);
+    it->d_cw = (CompressWord *)dalloc(it->h_cw.size() * sizeof(CompressWord));
+    it->d_pend = (Pend *)dalloc(it->pcap * sizeof(Pend));
+    it->d_cand = (Cand *)dalloc(it->cap * sizeof(Cand));
+    it->d_state = (ScanState *)dalloc(sizeof(ScanState));
+    if (!it->d_layout || !it->d_cw || !it->d_pend || !it->d_cand || !it->d_state) {
+        drop(it, gepi_fail(GEPI_ERR_NOMEM, "device memory for a scan of %lld variants",
+                           (long long)it->n_var));
+        return it->status;
+    }
+    it->footprint = round_block(lay) + round_block(it->h_cw.size() * sizeof(CompressWord)) +
+                    round_block(it->pcap * sizeof(Pend)) + round_block(it->cap * sizeof(Cand)) +
+                    round_block(sizeof(ScanState));
+    memset(&it->h_state0, 0, sizeof it->h_state0);
+    it->h_state0.thr_score = INFINITY;
+    it->h_state0.thr_idx = ~0ull;
+    HIP_TRY(hipMemcpyAsync(it->d_cw, it->h_cw.data(), it->h_cw.size() * sizeof(CompressWord),
+                           hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(it->d_state, &it->h_state0, sizeof(ScanState), hipMemcpyHostToDevice,
+                           stream));
+    if (it->n_k4 > 0) {
+        const int64_t rows = it->n_var_pad * 3;
+        k_class_f4<<<(unsigned)((rows + 255) / 256), 256, 0, stream>>>(
+            p->d_nat, (int32_t)it->n_var, p->w64, it->d_cw, rows, it->ks0, it->d_layout);
+        HIP_TRY(hipGetLastError());
+    }
+    it->max_band = std::max<int64_t>(64, (int64_t)(2.0e9 / (double)F4_TILE_EVALS));
+    it->band = std::max<int64_t>(1, std::min<int64_t>(it->max_band, (2 * it->k) / F4_TILE_EVALS + 1));
+    it->forced_band = 0;
+    if (const char *env = getenv("GEPI_BAND_TILES")) {
+        it->forced_band = std::max<long long>(1, atoll(env));
+        it->band = it->forced_band;
+    }
+    it->t0 = 0;
+    it->count = 0;
+    it->exact_total = 0;
+    active_.push_back(it);
+    return GEPI_OK;
+}
+
+int Engine::finish(ScanItem *it) {
+    std::vector<Cand> host((size_t)it->count);
+    if (it->count > 0)
+        HIP_TRY(hipMemcpyAsync(host.data(), it->d_cand, sizeof(Cand) * it->count,
+                               hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    it->results.resize((size_t)it->k);
+    const int64_t n = candidates_to_results(host, it->k, it->results.data());
+    it->results.resize((size_t)n);
+    it->stats.exact_evals = (int64_t)it->exact_total;
+    drop(it, n == it->k ? GEPI_OK
+                        : gepi_fail(GEPI_ERR_INTERNAL, "top-k extraction returned %lld of %lld",
+                                    (long long)n, (long long)it->k));
+    return GEPI_OK;
+}
+
+int Engine::round(std::vector<ScanItem *> &finished) {
+    if (active_.empty()) return GEPI_OK;
+    const size_t na = active_.size();
+    int rc;
+    if ((rc = grow_desc(na)) != GEPI_OK) return rc;
+    /* bands, scaled down uniformly when the round's workgroup budget is set */
+    int64_t total_wg = 0;
+    for (ScanItem *it : active_) {
+        it->band_used = std::min<int64_t>(it->band, it->n_tiles - it->t0);
+        total_wg += it->band_used;
+    }
+    if (round_wg_budget > 0 && total_wg > round_wg_budget) {
+        const double f = (double)round_wg_budget / (double)total_wg;
+        for (ScanItem *it : active_)
+            it->band_used = std::max<int64_t>(1, (int64_t)((double)it->band_used * f));
+    }
+    std::vector<BatchDesc> bd[2];
+    int64_t wg[2] = {0, 0};
+    std::vector<ExactDesc> ed(na);
+    std::vector<TopkDesc> td(na);
+    for (size_t i = 0; i < na; i++) {
+        ScanItem *it = active_[i];
+        BatchDesc d;
+        memset(&d, 0, sizeof d);
+        d.g.planes_k = reinterpret_cast<const uint64_t *>(it->d_layout);
+        d.g.tile_begin = it->t0;
+        d.g.n_tiles = (int32_t)it->band_used;
+        d.g.n_var = (int32_t)it->n_var;
+        d.g.n_var_pad = (int32_t)it->n_var_pad;
+        d.g.n_ks = it->n_k4;
+        d.g.ks0 = it->ks0;
+        d.g.st = it->d_state;
+        d.g.pend = it->d_pend;
+        d.g.pcap = it->pcap;
+        d.g.lfact = tabs_.lfact;
+        const int f = it->ffun == GEPI_FFUNC_ENTROPY ? 0 : 1;
+        d.wg_begin = wg[f];
+        wg[f] += it->band_used;
+        bd[f].push_back(d);
+        ExactDesc &e = ed[i];
+        memset(&e, 0, sizeof e);
+        e.e.st = it->d_state;
+        e.e.pend = it->d_pend;
+        e.e.pcap = it->pcap;
+        e.e.cand = it->d_cand;
+        e.e.cap = it->cap;
+        e.e.lhi = tabs_.lhi;
+        e.e.llo = tabs_.llo;
+        e.ffun = it->ffun;
+        TopkDesc &t = td[i];
+        t.st = it->d_state;
+        t.cand = it->d_cand;
+        t.cap = it->cap;
+        t.drop_t0 = it->t0;
+        t.drop_t1 = it->t0 + it->band_used;
+        t.report = d_rep_ + i;
+        t.k = (uint32_t)it->k;
+        t.ffun = it->ffun;
+    }
+    for (int f = 0; f < 2; f++)
+        if (!bd[f].empty())
+            HIP_TRY(hipMemcpyAsync(d_bdesc_[f], bd[f].data(), sizeof(BatchDesc) * bd[f].size(),
+                                   hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(d_edesc_, ed.data(), sizeof(ExactDesc) * na, hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipMemcpyAsync(d_tdesc_, td.data(), sizeof(TopkDesc) * na, hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipEventRecord(ev0_, stream));
+    if (wg[0] > 0)
+        k_count_f4_batch<GEPI_FFUNC_ENTROPY><<<(unsigned)wg[0], F4_THREADS, 0, stream>>>(
+            d_bdesc_[0], (int32_t)bd[0].size(), (int32_t)wg[0]);
+    if (wg[1] > 0)
+        k_count_f4_batch<GEPI_FFUNC_FISHERPV><<<(unsigned)wg[1], F4_THREADS, 0, stream>>>(
+            d_bdesc_[1], (int32_t)bd[1].size(), (int32_t)wg[1]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev1_, stream));
+    k_exact_batch<<<dim3(EXACT_BLOCKS_X, (unsigned)na), 256, 0, stream>>>(d_edesc_);
+    k_topk_wg<<<(unsigned)na, TOPK_THREADS, 0, stream>>>(d_tdesc_);
+    HIP_TRY(hipGetLastError());
+    std::vector<ItemReport> rep(na);
+    HIP_TRY(hipMemcpyAsync(rep.data(), d_rep_, sizeof(ItemReport) * na, hipMemcpyDeviceToHost,
+                           stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, ev0_, ev1_);
+    ms_count += ms;
+    rounds++;
+
+    std::vector<ScanItem *> still;
+    still.reserve(na);
+    for (size_t i = 0; i < na; i++) {
+        ScanItem *it = active_[i];
+        const ItemReport &r = rep[i];
+        const int64_t used = it->band_used;
+        it->stats.bands++;
+        if (r.pend_overflow) {
+            it->stats.pend_overflows++;
+            it->band = std::max<int64_t>(1, used / 4);
+            still.push_back(it);
+            continue;
+        }
+        if (r.cand_overflow) {
+            it->stats.cand_overflows++;
+            it->count = r.count;
+            it->band = std::max<int64_t>(1, used / 4);
+            still.push_back(it);
+            continue;
+        }
+        it->exact_total += r.pend_count;
+        const unsigned long long added = r.raw_count - it->count;
+        it->count = r.count;
+        const double rate = (double)std::max<unsigned long long>(r.pend_count, added) / (double)used;
+        const int64_t room = (int64_t)(it->cap - (unsigned long long)it->k) / 2;
+        const int64_t next = rate > 0.0 ? (int64_t)((double)room / rate) : it->max_band;
+        it->band = std::max<int64_t>(1, std::min<int64_t>({it->max_band, next, used * 4}));
+        if (it->forced_band) it->band = std::min<int64_t>(it->forced_band, used * 4);
+        it->t0 += used;
+        if (it->t0 >= it->n_tiles) {
+            if ((rc = finish(it)) != GEPI_OK) return rc;
+            finished.push_back(it);
+        } else {
+            still.push_back(it);
+        }
+    }
+    active_.swap(still);
+    return GEPI_OK;
+}
+
+} // namespace gepi_host
+
+using namespace gepi_host;
+
+struct gepi_engine {
+    Engine eng;
+};
+
+extern "C" int gepi_engine_create(int32_t device, gepi_engine **out) {
+    if (!out) return gepi_fail(GEPI_ERR_ARG, "null out");
+    *out = nullptr;
+    gepi_engine *e = new gepi_engine();
+    int rc = e->eng.init(device);
+    if (rc != GEPI_OK) {
+        e->eng.release();
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return GEPI_OK;
+}
+
+extern "C" int gepi_engine_destroy(gepi_engine *e) {
+    if (!e) return GEPI_OK;
+    e->eng.release();
+    delete e;
+    return GEPI_OK;
+}
+
+extern "C" int gepi_engine_scan_batch(gepi_engine *e, gepi_batch_item *items, int32_t n_items,
+                                      gepi_batch_stats *stats) {
+    if (!e) return gepi_fail(GEPI_ERR_ARG, "null engine");
+    if (n_items < 0 || (n_items > 0 && !items)) return gepi_fail(GEPI_ERR_ARG, "bad item list");
+    int rc;
+    if ((rc = set_device(e->eng.device)) != GEPI_OK) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    gepi_batch_stats bs;
+    memset(&bs, 0, sizeof bs);
+    std::vector<ScanItem> its((size_t)n_items);
+    e->eng.ms_count = 0.0;
+    e->eng.rounds = 0;
+    for (int32_t i = 0; i < n_items; i++) {
+        ScanItem &it = its[(size_t)i];
+        it.panel = items[i].panel;
+        it.mask_ctrl = items[i].mask_ctrl;
+        it.mask_case = items[i].mask_case;
+        it.topmost = items[i].topmost;
+        it.ffun = items[i].ffun;
+        items[i].n_tops = 0;
+        if (it.topmost > 0 && !items[i].tops) {
+            it.status = gepi_fail(GEPI_ERR_ARG, "null tops");
+            it.done = true;
+            continue;
+        }
+        e->eng.add(&it);
+    }
+    HIP_TRY(hipStreamSynchronize(e->eng.stream));
+    bs.ms_prepare = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                        .count();
+    std::vector<ScanItem *> fin;
+    rc = GEPI_OK;
+    while (!e->eng.idle()) {
+        if ((rc = e->eng.round(fin)) != GEPI_OK) break;
+    }
+    for (int32_t i = 0; i < n_items; i++) {
+        ScanItem &it = its[(size_t)i];
+        if (!it.done) it.status = rc != GEPI_OK ? rc : GEPI_ERR_INTERNAL;
+        items[i].status = it.status;
+        items[i].stats = it.stats;
+        if (it.status == GEPI_OK) {
+            const int64_t n = (int64_t)it.results.size();
+            if (n > 0) memcpy(items[i].tops, it.results.data(), sizeof(gepi_result) * (size_t)n);
+            items[i].n_tops = n;
+        } else {
+            bs.items_failed++;
+        }
+        bs.evals += it.stats.evals;
+    }
+    bs.ms_count_kernels = e->eng.ms_count;
+    bs.rounds = e->eng.rounds;
+    bs.ms_batch = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                      .count();
+    if (stats) *stats = bs;
+    return rc;
+}

@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
         src[u] = h < 384 ? srcA + h : srcB + (h - 384);
         dst[u] = (h >> 1) * MF_ROW + 32 * (h & 1);
     }
-    const int last = g.n_ks - 1;
+    const int last = g.n_ks > 0 ? g.n_ks - 1 : 0; /* n_ks == 0: no K loop, step 0 exists */
     uint32_t pf1[3], pf2[3];
     auto shift_of = [&](int s) { return ((s < g.ks0) == (g.cls0_high != 0)) ? 7 : 0; };
 
@@ -324,13 +324,15 @@ __device__ __forceinline__ v8i expand_f4(uint32_t w) {
                (int32_t)((w >> 2) & 0x11111111u), (int32_t)((w >> 3) & 0x11111111u), 0, 0, 0, 0};
 }
 
+/* One 64 x 32-variant workgroup tile of the fp4 path (tile numbering as in
+ * tile_of_f4).  Called by every thread of a 256-thread workgroup.          */
 template <int FFUN>
-__global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
+__device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_Q];
     __shared__ PendStage stage;
     stage_init(stage);
     int32_t ta, tb;
-    tile_of_f4(band_tile(g.tile_begin, g.n_tiles), ta, tb);
+    tile_of_f4(tile, ta, tb);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
                                     : base + (int64_t)tb * (TB4 * 3) + (tid - F4_QA);
     const bool has1 = tid < F4_Q - F4_THREADS;
     const uint4 *src1 = base + (int64_t)tb * (TB4 * 3) + (F4_THREADS - F4_QA) + (has1 ? tid : 0);
-    const int last = g.n_ks - 1;
+    const int last = g.n_ks > 0 ? g.n_ks - 1 : 0; /* n_ks == 0: no K loop, step 0 exists */
 
     lds[0][tid] = src0[0];
     if (has1) lds[0][F4_THREADS + tid] = src1[0];
@@ -437,6 +439,11 @@ __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
         }
     }
     stage_flush(g, stage);
+}
+
+template <int FFUN>
+__global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
+    count_f4_tile<FFUN>(g, band_tile(g.tile_begin, g.n_tiles));
 }
 
 } // namespace gepi

@@ -1,0 +1,103 @@
+/*
+ * gepi_engine.h -- the batched scan engine (host side).  Scans are added at
+ * any time and advance together in rounds; a round is one count launch per
+ * objective, one exact-score launch and one top-k launch for all active
+ * scans, then a single read of their reports.  The batch C ABI runs rounds
+ * until every scan is done; the serving loop admits new requests between
+ * rounds.  All device work runs on the engine's stream.
+ */
+#ifndef GEPI_ENGINE_H
+#define GEPI_ENGINE_H
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <map>
+#include <vector>
+
+#include "../../include/gepicount.h"
+#include "gepi_host.h"
+
+namespace gepi {
+struct BatchDesc;
+struct ExactDesc;
+struct TopkDesc;
+struct ItemReport;
+} // namespace gepi
+
+namespace gepi_host {
+
+/* One scan inside the engine. */
+struct ScanItem {
+    /* request (masks are read only by Engine::add) */
+    gepi_panel *panel = nullptr;
+    const uint64_t *mask_ctrl = nullptr, *mask_case = nullptr;
+    int64_t topmost = 0;
+    int32_t ffun = GEPI_FFUNC_ENTROPY;
+    void *tag = nullptr; /* caller's */
+    /* outcome */
+    int32_t status = GEPI_OK;
+    bool done = false;
+    std::vector<gepi_result> results;
+    gepi_scan_stats stats{};
+    std::chrono::steady_clock::time_point t_added;
+    /* scan state */
+    int64_t k = 0, n_var = 0, n_var_pad = 0, n_tiles = 0, t0 = 0;
+    int64_t band = 1, band_used = 0, max_band = 1, forced_band = 0;
+    int32_t n0 = 0, n1 = 0, n_k4 = 0, ks0 = 0;
+    unsigned long long cap = 0, pcap = 0, count = 0, exact_total = 0;
+    uint4 *d_layout = nullptr;
+    gepi::CompressWord *d_cw = nullptr;
+    gepi::Pend *d_pend = nullptr;
+    gepi::Cand *d_cand = nullptr;
+    gepi::ScanState *d_state = nullptr;
+    size_t footprint = 0; /* device bytes held */
+    std::vector<gepi::CompressWord> h_cw;
+    gepi::ScanState h_state0;
+};
+
+class Engine {
+  public:
+    int init(int device);
+    void release();
+    /* Validates and stages a scan (class layout launched on the stream).  A
+     * scan that fails validation gets its status set and done = true.       */
+    int add(ScanItem *it);
+    /* One round over the active scans; completed scans (results filled,
+     * device memory returned) are appended to `finished`.                    */
+    int round(std::vector<ScanItem *> &finished);
+    bool idle() const { return active_.empty(); }
+    size_t active_count() const { return active_.size(); }
+    /* Device bytes a scan of this panel would hold (admission control). */
+    static size_t estimate_footprint(const gepi_panel *p, int64_t topmost);
+    size_t bytes_in_use() const { return in_use_; }
+    /* Workgroups per count round (0 = unlimited): bounds round latency. */
+    int64_t round_wg_budget = 0;
+    double ms_count = 0.0;
+    int32_t rounds = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+  private:
+    void *dalloc(size_t bytes);
+    void dfree(void *p);
+    int finish(ScanItem *it);
+    void drop(ScanItem *it, int status);
+    int grow_desc(size_t n);
+
+    DeviceTables tabs_;
+    std::vector<ScanItem *> active_;
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> live_;
+    size_t in_use_ = 0;
+    gepi::BatchDesc *d_bdesc_[2] = {nullptr, nullptr};
+    gepi::ExactDesc *d_edesc_ = nullptr;
+    gepi::TopkDesc *d_tdesc_ = nullptr;
+    gepi::ItemReport *d_rep_ = nullptr;
+    size_t desc_cap_ = 0;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+} // namespace gepi_host
+
+#endif

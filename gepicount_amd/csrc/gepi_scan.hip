@@ -42,10 +42,12 @@
 #include "gepi_score.h"
 #include "gepi_stage.h"
 #include "gepi_topk.h"
+#include "gepi_host.h"
 
 #define GEPI_VERSION "gepicount-mi355x 0.2.0"
 
 using namespace gepi;
+using namespace gepi_host;
 
 /* ------------------------------------------------------------------------ */
 /* errors                                                                    */
@@ -62,50 +64,10 @@ int gepi_fail(int code, const char *fmt, ...) {
 extern "C" const char *gepi_last_error(void) { return g_err; }
 extern "C" const char *gepi_version(void) { return GEPI_VERSION; }
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return gepi_fail(e_ == hipErrorOutOfMemory ? GEPI_ERR_NOMEM : GEPI_ERR_DEVICE, \
-                             "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),        \
-                             __FILE__, __LINE__);                                         \
-    } while (0)
-
 /* ------------------------------------------------------------------------ */
-/* panel                                                                     */
+/* device, tables                                                            */
 
-struct DeviceTables {
-    int32_t tmax = -1;
-    double *lhi = nullptr, *llo = nullptr, *lfact = nullptr;
-};
-
-struct gepi_panel {
-    int device = 0;
-    int64_t n_var = 0;
-    int32_t n_samples = 0;
-    int32_t w64 = 0;
-    uint64_t *d_nat = nullptr;
-    hipStream_t stream = nullptr;
-    /* per-scan workspace, grown on demand and reused */
-    uint32_t *d_cls = nullptr;
-    size_t cls_bytes = 0;
-    uint64_t *d_clsk = nullptr;
-    size_t clsk_bytes = 0;
-    int32_t *d_perm = nullptr;
-    size_t perm_bytes = 0;
-    CompressWord *d_cw = nullptr; /* fp4 path: class compression words */
-    size_t cw_bytes = 0;
-    Cand *d_cand[2] = {nullptr, nullptr};
-    Pend *d_pend = nullptr;
-    unsigned long long buf_cap = 0;  /* records in each candidate buffer */
-    unsigned long long pend_cap = 0; /* records in the pending list       */
-    ScanState *d_state = nullptr;
-    uint64_t *d_small = nullptr;
-    DeviceTables tabs;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-};
-
-namespace {
+namespace gepi_host {
 
 int set_device(int device) {
     int n = 0;
@@ -124,35 +86,31 @@ int set_device(int device) {
     return GEPI_OK;
 }
 
-int ensure_tables(gepi_panel *p, int32_t tmax) {
-    if (tmax <= p->tabs.tmax) return GEPI_OK;
+int ensure_tables(DeviceTables &tabs, int32_t tmax) {
+    if (tmax <= tabs.tmax) return GEPI_OK;
     int32_t n = tmax + 1;
     std::vector<double> hi(n), lo(n), lf(n);
     gepi_build_tables(n, hi.data(), lo.data(), lf.data());
-    hipFree(p->tabs.lhi);
-    hipFree(p->tabs.llo);
-    hipFree(p->tabs.lfact);
-    p->tabs = DeviceTables();
-    HIP_TRY(hipMalloc(&p->tabs.lhi, sizeof(double) * n));
-    HIP_TRY(hipMalloc(&p->tabs.llo, sizeof(double) * n));
-    HIP_TRY(hipMalloc(&p->tabs.lfact, sizeof(double) * n));
-    HIP_TRY(hipMemcpy(p->tabs.lhi, hi.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->tabs.llo, lo.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->tabs.lfact, lf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    p->tabs.tmax = tmax;
+    hipFree(tabs.lhi);
+    hipFree(tabs.llo);
+    hipFree(tabs.lfact);
+    tabs = DeviceTables();
+    HIP_TRY(hipMalloc(&tabs.lhi, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&tabs.llo, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&tabs.lfact, sizeof(double) * n));
+    HIP_TRY(hipMemcpy(tabs.lhi, hi.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tabs.llo, lo.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tabs.lfact, lf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    tabs.tmax = tmax;
     return GEPI_OK;
 }
 
-template <typename T>
-int grow(T **ptr, size_t *have, size_t need) {
-    if (need <= *have && *ptr) return GEPI_OK;
-    hipFree(*ptr);
-    *ptr = nullptr;
-    *have = 0;
-    HIP_TRY(hipMalloc(ptr, need));
-    *have = need;
-    return GEPI_OK;
-}
+} // namespace gepi_host
+
+/* ------------------------------------------------------------------------ */
+/* panel                                                                     */
+
+namespace {
 
 int panel_alloc(gepi_panel *p, int device, int64_t n_var, int32_t n_samples) {
     p->device = device;
@@ -313,7 +271,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const int64_t pairs_per_tile = path == 3 ? (int64_t)TM * TB4 : (int64_t)tv * tv;
     const int32_t n_ks = path == 3 ? wused / 4 : wused / 2;
 
-    if ((rc = ensure_tables(p, n0 + n1 + 1)) != GEPI_OK) return rc;
+    if ((rc = ensure_tables(p->tabs, n0 + n1 + 1)) != GEPI_OK) return rc;
     if (path != 3 &&
         (rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK)
         return rc;
@@ -326,19 +284,8 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
      * class (mask bits at or beyond n_samples are dropped) */
     std::vector<CompressWord> cw;
     if (path == 3) {
-        const int32_t nw32 = 2 * p->w64;
-        cw.resize((size_t)2 * std::max(nw32, 1));
-        for (int c = 0; c < 2; c++) {
-            const uint64_t *mk = c == 0 ? mask_ctrl : mask_case;
-            for (int32_t j = 0; j < nw32; j++) {
-                uint32_t m = 0;
-                if (mk) m = (uint32_t)(mk[j >> 1] >> (32 * (j & 1)));
-                const int32_t lo = 32 * j;
-                if (lo >= p->n_samples) m = 0;
-                else if (p->n_samples - lo < 32) m &= (1u << (p->n_samples - lo)) - 1u;
-                cw[(size_t)c * nw32 + j] = make_compress_word(m);
-            }
-        }
+        int32_t c0n, c1n;
+        build_compress_words(mask_ctrl, mask_case, p->n_samples, p->w64, cw, c0n, c1n);
         if ((rc = grow(&p->d_cw, &p->cw_bytes, cw.size() * sizeof(CompressWord))) != GEPI_OK)
             return rc;
     }
@@ -559,24 +506,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         HIP_TRY(hipMemcpyAsync(host.data(), p->d_cand[cur], sizeof(Cand) * count,
                                hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
-    std::sort(host.begin(), host.end(), [](const Cand &a, const Cand &b) {
-        uint64_t ha = order_bits(a.score), hb = order_bits(b.score);
-        return ha < hb || (ha == hb && a.idx < b.idx);
-    });
-    const int64_t n = std::min<int64_t>((int64_t)host.size(), k);
-    for (int64_t i = 0; i < n; i++) {
-        const Cand &c = host[(size_t)i];
-        gepi_result &r = tops[i];
-        r.sA = (int64_t)(c.idx >> 32);
-        r.sB = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull);
-        r.fun = (int32_t)(c.idx & 7);
-        r.w = (int32_t)c.w;
-        r.x = (int32_t)c.x;
-        r.y = (int32_t)c.y;
-        r.z = (int32_t)c.z;
-        r.obj = c.score;
-        r.reserved = 0;
-    }
+    const int64_t n = candidates_to_results(host, k, tops);
     *n_tops = n;
     ms_top += ms_since(ts);
     st_local.ms_count_kernels = ms_count;

@@ -25,7 +25,7 @@
 
 namespace gepi {
 
-__global__ void k_decode_bed(const uint8_t *__restrict__ bed, int64_t n_var, int32_t n_samples,
+static __global__ void k_decode_bed(const uint8_t *__restrict__ bed, int64_t n_var, int32_t n_samples,
                              int32_t w64, int64_t bps, uint64_t *__restrict__ nat) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n_var * w64) return;
@@ -53,7 +53,7 @@ __global__ void k_decode_bed(const uint8_t *__restrict__ bed, int64_t n_var, int
 }
 
 /* out[v][p][d] bit b = panel bit perm[32*d + b] (0 for perm < 0 or v >= n_var). */
-__global__ void __launch_bounds__(256) k_gather_class(const uint64_t *__restrict__ nat,
+static __global__ void __launch_bounds__(256) k_gather_class(const uint64_t *__restrict__ nat,
                                                        int32_t n_var, int32_t w64,
                                                        const int32_t *__restrict__ perm,
                                                        int32_t wc, int32_t wused,
@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_gather_class(const uint64_t *__restrict
 }
 
 /* Tile of 32 variants x 32 K steps per workgroup. */
-__global__ void __launch_bounds__(256) k_transpose_k(const uint32_t *__restrict__ cls,
+static __global__ void __launch_bounds__(256) k_transpose_k(const uint32_t *__restrict__ cls,
                                                       int32_t n_var_pad, int32_t wc, int32_t n_ks,
                                                       uint64_t *__restrict__ out) {
     __shared__ uint64_t tile[96][33];
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_transpose_k(const uint32_t *__restrict_
 }
 
 /* rows = n_var_pad*3 (a multiple of 64); grid (rows / 64, ceil(n_k4 / 16)). */
-__global__ void __launch_bounds__(256) k_transpose_f4(const uint32_t *__restrict__ cls,
+static __global__ void __launch_bounds__(256) k_transpose_f4(const uint32_t *__restrict__ cls,
                                                        int32_t rows, int32_t wc, int32_t n_k4,
                                                        uint4 *__restrict__ out) {
     __shared__ uint4 tile[64][17];
@@ -219,7 +219,7 @@ struct BitStream {
  * class 1 steps [ks0, ks0 + ceil(n1 / 128)).  The panel rows of a
  * workgroup are staged through LDS 8 words at a time (coalesced reads).     */
 constexpr int CLS_CHUNK = 8;
-__global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
+static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
                                                    int32_t w64, const CompressWord *__restrict__ cw,
                                                    int64_t rows, int32_t ks0,
                                                    uint4 *__restrict__ out) {

@@ -25,16 +25,17 @@ struct ExactArgs {
     const double *lhi, *llo;
 };
 
-/* Exact score of every pending evaluation; admits keys <= the threshold. */
+/* Exact score of pending evaluations [start, n) with the given stride;
+ * admits keys <= the threshold.  Nothing when the pending list overflowed. */
 template <int FFUN>
-__global__ void __launch_bounds__(256) k_exact(ExactArgs e) {
+__device__ __forceinline__ void exact_range(const ExactArgs &e, unsigned long long start,
+                                            unsigned long long stride) {
     if (e.st->pend_overflow) return;
     unsigned long long n = e.st->pend_count;
     if (n > e.pcap) n = e.pcap;
     const double thr_s = e.st->thr_score;
     const uint64_t thr_i = e.st->thr_idx;
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (unsigned long long)gridDim.x * blockDim.x) {
+    for (unsigned long long i = start; i < n; i += stride) {
         const Pend p = e.pend[i];
         const int32_t w = p.w, x = p.x, y = p.y, z = p.z;
         const double s = (FFUN == GEPI_FFUNC_ENTROPY) ? entropy_exact(w, x, y, z, e.lhi, e.llo)
@@ -57,8 +58,14 @@ __global__ void __launch_bounds__(256) k_exact(ExactArgs e) {
     }
 }
 
+template <int FFUN>
+__global__ void __launch_bounds__(256) k_exact(ExactArgs e) {
+    exact_range<FFUN>(e, (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x,
+                      (unsigned long long)gridDim.x * blockDim.x);
+}
+
 /* Clears the pending list at the start of a band. */
-__global__ void k_reset_pending(ScanState *st) {
+static __global__ void k_reset_pending(ScanState *st) {
     if (threadIdx.x != 0) return;
     st->pend_count = 0;
     st->pend_overflow = 0;
@@ -90,7 +97,7 @@ __device__ __forceinline__ uint32_t key_digit(uint64_t hi, uint64_t lo, uint32_t
     return b < 64 ? (uint32_t)((hi >> (56 - b)) & 0xFF) : (uint32_t)((lo >> (120 - b)) & 0xFF);
 }
 
-__global__ void __launch_bounds__(256) k_sel_hist(const Cand *__restrict__ cand, ScanState *st) {
+static __global__ void __launch_bounds__(256) k_sel_hist(const Cand *__restrict__ cand, ScanState *st) {
     if (st->done) return;
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
@@ -106,7 +113,7 @@ __global__ void __launch_bounds__(256) k_sel_hist(const Cand *__restrict__ cand,
     if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], h[threadIdx.x]);
 }
 
-__global__ void k_sel_pick(ScanState *st) {
+static __global__ void k_sel_pick(ScanState *st) {
     if (threadIdx.x != 0 || st->done) return;
     uint32_t cum = 0, d = 0;
     for (d = 0; d < 256; d++) {
@@ -133,7 +140,7 @@ __global__ void k_sel_pick(ScanState *st) {
         st->done = 2;
 }
 
-__global__ void __launch_bounds__(256) k_sel_gather(const Cand *__restrict__ cand, ScanState *st,
+static __global__ void __launch_bounds__(256) k_sel_gather(const Cand *__restrict__ cand, ScanState *st,
                                                      uint64_t *small) {
     if (st->done != 2) return;
     const SelPrefix s = load_prefix(st);
@@ -151,7 +158,7 @@ __global__ void __launch_bounds__(256) k_sel_gather(const Cand *__restrict__ can
     }
 }
 
-__global__ void __launch_bounds__(1024) k_sel_small(ScanState *st, const uint64_t *small) {
+static __global__ void __launch_bounds__(1024) k_sel_small(ScanState *st, const uint64_t *small) {
     if (st->done != 2) return;
     __shared__ uint64_t kh[SEL_SMALL], kl[SEL_SMALL];
     const uint32_t n = st->gather_n < SEL_SMALL ? st->gather_n : SEL_SMALL;
@@ -187,7 +194,7 @@ __global__ void __launch_bounds__(1024) k_sel_small(ScanState *st, const uint64_
 
 /* Keeps records with key <= the selected key; records of workgroup tiles in
  * [drop_t0, drop_t1) (geometry tv, rect as in tile_index) are dropped too.  */
-__global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, ScanState *st,
+static __global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, ScanState *st,
                                                   Cand *__restrict__ dst, int64_t drop_t0,
                                                   int64_t drop_t1, int32_t tv, int32_t rect) {
     const uint64_t khi = st->key_hi, klo = st->key_lo;
@@ -210,15 +217,12 @@ __global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, S
     }
 }
 
-/* After a compaction: adopt the compacted buffer and the new threshold. */
-__global__ void k_finalize(ScanState *st, int32_t ffun) {
-    if (threadIdx.x != 0) return;
-    st->count = st->count2;
-    st->count2 = 0;
-    st->overflow = 0;
-    double thr = from_order_bits(st->key_hi);
+/* Threshold score and filter coefficient of a selected key. */
+__device__ __forceinline__ void adopt_threshold(ScanState *st, uint64_t key_hi, uint64_t key_lo,
+                                                int32_t ffun) {
+    const double thr = from_order_bits(key_hi);
     st->thr_score = thr;
-    st->thr_idx = st->key_lo;
+    st->thr_idx = key_lo;
     float kf = 0.0f;
     if (ffun == GEPI_FFUNC_ENTROPY) {
         double it = -thr - 1e-12;
@@ -231,8 +235,17 @@ __global__ void k_finalize(ScanState *st, int32_t ffun) {
     st->filt_k = kf;
 }
 
+/* After a compaction: adopt the compacted buffer and the new threshold. */
+static __global__ void k_finalize(ScanState *st, int32_t ffun) {
+    if (threadIdx.x != 0) return;
+    st->count = st->count2;
+    st->count2 = 0;
+    st->overflow = 0;
+    adopt_threshold(st, st->key_hi, st->key_lo, ffun);
+}
+
 /* Resets the select state for a new select of the rank-th smallest key. */
-__global__ void k_sel_reset(ScanState *st, uint32_t rank) {
+static __global__ void k_sel_reset(ScanState *st, uint32_t rank) {
     if (threadIdx.x != 0) return;
     st->key_hi = 0;
     st->key_lo = 0;

@@ -133,6 +133,45 @@ int gepi_scan(gepi_panel *panel, const uint64_t *mask_ctrl, const uint64_t *mask
 /* Reads the panel's planes back in reader layout (testing aid). */
 int gepi_panel_download(gepi_panel *panel, uint64_t *bit_data, uint64_t *valid_data);
 
+/* ---- batched scans ---------------------------------------------------------
+ * Many scans (panels of any size, phenotype vectors) advance together in
+ * rounds of three launches, so small scans share the GPU instead of
+ * running one after another.  No counterpart in the reference, which runs
+ * one process_gpu_1 per process (src/gEpiCount.cpp:84-87); each item's
+ * result is identical to gepi_scan's on the same input (fp4 count path).
+ * An engine owns a stream and device workspace on one device; one batch at a
+ * time per engine, and an item's panel must not be scanned elsewhere while
+ * the batch runs.                                                          */
+typedef struct gepi_engine gepi_engine;
+
+typedef struct gepi_batch_item {
+    gepi_panel *panel;          /* resident panel on the engine's device        */
+    const uint64_t *mask_ctrl;  /* [ceil(n_samples/64)] class 0 (phenotype 1)   */
+    const uint64_t *mask_case;  /* class 1 (phenotype 2)                        */
+    int64_t topmost;            /* results wanted                               */
+    int32_t ffun;               /* GEPI_FFUNC_ENTROPY or GEPI_FFUNC_FISHERPV    */
+    int32_t status;             /* out: GEPI_OK or this item's error code       */
+    gepi_result *tops;          /* out: [topmost], caller owned                 */
+    int64_t n_tops;             /* out                                          */
+    gepi_scan_stats stats;      /* out: ms_scan = batch start to item completion */
+} gepi_batch_item;
+
+typedef struct gepi_batch_stats {
+    double ms_batch;            /* host wall time of the batch                  */
+    double ms_prepare;          /* class layouts of all items                   */
+    double ms_count_kernels;    /* count launches (HIP events)                  */
+    int64_t evals;              /* summed over items                            */
+    int32_t rounds;             /* (count, exact, top-k) rounds                 */
+    int32_t items_failed;
+} gepi_batch_stats;
+
+int gepi_engine_create(int32_t device, gepi_engine **out);
+int gepi_engine_destroy(gepi_engine *engine);
+/* Runs every item to completion.  Returns GEPI_OK when the batch ran (an
+ * item's own failure is in its status) or the device error that stopped it. */
+int gepi_engine_scan_batch(gepi_engine *engine, gepi_batch_item *items, int32_t n_items,
+                           gepi_batch_stats *stats);
+
 /* ---- host-side PLINK reader and report ---------------------------------- */
 /* Replaces BinPlinkReader<ui8> (src/reader/BinPlinkReader.h, PlinkReader.h).
  * Fields are read-only for the caller; release with gepi_plink_close.        */

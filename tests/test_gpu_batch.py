@@ -1,0 +1,141 @@
+"""Batched scans (gepi_engine_scan_batch): every item's result must be
+identical to the same scan run alone (Panel.scan on the fp4 path) and to the
+oracle's gold scan, whatever the other items in the batch."""
+import numpy as np
+import pytest
+
+import gepicount_amd as ge
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("sA", "sB", "fun", "w", "x", "y", "z", "obj")
+
+
+def assert_same(got, ref, label=""):
+    assert len(got) == len(ref), (label, len(got), len(ref))
+    for k in FIELDS:
+        if not np.array_equal(got[k], ref[k]):
+            bad = int(np.flatnonzero(got[k] != ref[k])[0])
+            raise AssertionError("%s: field %s differs first at rank %d: got %r ref %r" % (
+                label, k, bad, got[bad], ref[bad]))
+
+
+def make_mix(seed, n_items, vmax=300, nmax=1200):
+    """Uneven scans: log-uniform variants and samples, case fraction U[0.2, 0.8],
+    both objectives, several items sharing one panel with new phenotypes."""
+    rng = np.random.default_rng(seed)
+    panels, items = [], []
+    for i in range(n_items):
+        if panels and rng.random() < 0.3:
+            pi = int(rng.integers(0, len(panels)))       # re-scan of a resident panel
+        else:
+            n_var = int(np.exp(rng.uniform(np.log(2), np.log(vmax))))
+            n = int(np.exp(rng.uniform(np.log(20), np.log(nmax))))
+            codes = orc.random_codes(rng, n_var, n, float(rng.choice([0.0, 0.02])))
+            bits, valid = orc.planes_from_codes(codes)
+            panels.append((bits, valid, n))
+            pi = len(panels) - 1
+        bits, valid, n = panels[pi]
+        frac = rng.uniform(0.2, 0.8)
+        cls = np.where(rng.random(n) < frac, 1, 0)
+        cls[rng.random(n) < 0.05] = -1                   # excluded samples
+        m0, m1 = orc.masks_from_classes(cls)
+        R = bits.shape[0]
+        total = 5 * R * (R + 1) // 2
+        k = int(min(total, rng.choice([1, 17, 100, 1000, 5000])))
+        items.append(dict(panel_index=pi, mask_ctrl=m0, mask_case=m1, topmost=k,
+                          ffun=int(rng.random() < 0.25)))
+    return panels, items
+
+
+def run_mix(panels, items, engine):
+    handles = [ge.Panel.from_planes(b, v, n) for b, v, n in panels]
+    try:
+        batch = [dict(it, panel=handles[it["panel_index"]]) for it in items]
+        res, bstats = engine.scan_batch(batch)
+        alone = [handles[it["panel_index"]].scan(it["mask_ctrl"], it["mask_case"],
+                                                 topmost=it["topmost"], ffun=it["ffun"],
+                                                 count_path=3)[0] for it in items]
+    finally:
+        for h in handles:
+            h.close()
+    return res, bstats, alone
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_batch_items_equal_alone_and_oracle(seed):
+    print("case batch mix seed=%d" % seed, flush=True)
+    panels, items = make_mix(seed, 14)
+    with ge.Engine() as eng:
+        res, bstats, alone = run_mix(panels, items, eng)
+    assert bstats["items_failed"] == 0
+    assert bstats["rounds"] >= 1
+    for i, (it, (got, st, status)) in enumerate(zip(items, res)):
+        assert status == 0, (i, status)
+        assert st["count_path"] == 3
+        assert_same(got, alone[i], "item %d vs alone" % i)
+        bits, valid, n = panels[it["panel_index"]]
+        ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], n),
+                            it["topmost"], ffun=it["ffun"])
+        assert_same(got, ref, "item %d vs oracle" % i)
+
+
+def test_batch_repeated_on_one_engine_is_identical():
+    panels, items = make_mix(5, 8)
+    with ge.Engine() as eng:
+        first, _, _ = run_mix(panels, items, eng)
+        second, _, _ = run_mix(panels, items, eng)
+    for i in range(len(items)):
+        assert_same(second[i][0], first[i][0], "repeat item %d" % i)
+
+
+def test_batch_overflow_recovery(monkeypatch):
+    """Forced 8-tile bands and small candidate buffers make items overflow
+    their pending lists and candidate buffers inside the batch."""
+    panels, items = make_mix(9, 6, vmax=260, nmax=300)
+    for it in items:
+        it["topmost"] = 20000 if it["ffun"] == 0 else 2000
+        it["ffun"] = 0 if it["ffun"] == 0 else 1
+    monkeypatch.setenv("GEPI_CAND_CAP", "1")
+    monkeypatch.setenv("GEPI_PEND_CAP", "4000000")
+    monkeypatch.setenv("GEPI_BAND_TILES", "8")
+    with ge.Engine() as eng:
+        handles = [ge.Panel.from_planes(b, v, n) for b, v, n in panels]
+        res, _ = eng.scan_batch([dict(it, panel=handles[it["panel_index"]]) for it in items])
+        for h in handles:
+            h.close()
+    for v in ("GEPI_CAND_CAP", "GEPI_PEND_CAP", "GEPI_BAND_TILES"):
+        monkeypatch.delenv(v)
+    assert sum(st["cand_overflows"] for _, st, _ in res) > 0
+    for i, (it, (got, st, status)) in enumerate(zip(items, res)):
+        assert status == 0
+        bits, valid, n = panels[it["panel_index"]]
+        ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], n),
+                            it["topmost"], ffun=it["ffun"])
+        assert_same(got, ref, "overflow item %d" % i)
+
+
+def test_batch_failing_and_degenerate_items_leave_others_intact():
+    rng = np.random.default_rng(21)
+    codes = orc.random_codes(rng, 40, 300, 0.01)
+    bits, valid = orc.planes_from_codes(codes)
+    cls = rng.choice([0, 1], size=300)
+    m0, m1 = orc.masks_from_classes(cls)
+    z = np.zeros_like(m0)
+    with ge.Engine() as eng, ge.Panel.from_planes(bits, valid, 300) as p:
+        items = [dict(panel=p, mask_ctrl=m0, mask_case=m1, topmost=500, ffun=0),
+                 dict(panel=p, mask_ctrl=m0, mask_case=m1, topmost=500, ffun=7),   # bad objective
+                 dict(panel=p, mask_ctrl=z, mask_case=z, topmost=300, ffun=0),     # no samples
+                 dict(panel=p, mask_ctrl=m0, mask_case=m1, topmost=0, ffun=0),     # nothing wanted
+                 dict(panel=p, mask_ctrl=m1, mask_case=m0, topmost=500, ffun=1)]
+        res, bstats = eng.scan_batch(items)
+    assert bstats["items_failed"] == 1
+    assert res[1][2] == -1 and len(res[1][0]) == 0
+    assert res[3][2] == 0 and len(res[3][0]) == 0
+    for i in (0, 2, 4):
+        it = items[i]
+        ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], 300),
+                            it["topmost"], ffun=it["ffun"])
+        assert res[i][2] == 0
+        assert_same(res[i][0], ref, "item %d" % i)
