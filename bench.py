@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="large", choices=["small", "medium", "large"])
+    ap.add_argument("--workload", default="large", choices=["small", "medium", "large", "mix64"],
+                    help="mix64: 64 uneven scans, batched (gepi_engine) against sequential")
     ap.add_argument("--n-var", type=int, default=None, help="override variant count")
     ap.add_argument("--topk", type=int, default=10000)
     ap.add_argument("--ffun", type=int, default=0)
@@ -91,6 +92,83 @@ def cpu_baseline(panel, m0, m1, n_samples, target_s, topk, ffun):
                       "(%d evals) of the same panel, %.1f s" % (topk, v2, ev, dt)}
 
 
+def run_mix(args):
+    """64 uneven scans (variants logU[500, 50,000], samples logU[200, 5,000],
+    case fraction U[0.2, 0.8], top-100 each), panels resident: one batch on
+    the engine against the same scans run one after another.  A step is one
+    pass over all 64 scans."""
+    import numpy as np
+
+    import gepicount_amd as ge
+    from gepicount_amd import synth
+
+    t = time.perf_counter()
+    specs = synth.mix_specs()
+    panels, items, evals = [], [], 0
+    for sp in specs:
+        p = synth.make_panel(spec=sp)
+        m0, m1 = synth.masks_from_pheno(p["pheno"])
+        h = ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"])
+        panels.append(h)
+        items.append(dict(panel=h, mask_ctrl=m0, mask_case=m1, topmost=sp["topk"], ffun=args.ffun))
+        R = 2 * p["n_var"]
+        evals += 5 * R * (R + 1) // 2
+    gen_s = time.perf_counter() - t
+    eng = ge.Engine()
+
+    def sequential():
+        return [it["panel"].scan(it["mask_ctrl"], it["mask_case"], topmost=it["topmost"],
+                                 ffun=it["ffun"], count_path=3)[0] for it in items]
+
+    for _ in range(args.warmup):
+        sequential()
+        eng.scan_batch(items)
+    seq_t, bat_t, bstats, lat = [], [], [], []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        seq = sequential()
+        seq_t.append(time.perf_counter() - ts)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        res, bs = eng.scan_batch(items)
+        bat_t.append(time.perf_counter() - ts)
+        bstats.append(bs)
+        lat.extend(st["ms_scan"] for _, st, _ in res)
+    total = time.perf_counter() - t0
+    identical = all(np.array_equal(r[0], q) for r, q in zip(res, seq))
+    k_ms = statistics.mean(b["ms_count_kernels"] for b in bstats)
+    var_pairs_samples = sum((sp["n_var"] * (sp["n_var"] + 1) // 2) * sp["n_samples"] for sp in specs)
+    ops = var_pairs_samples * 9 * 2.0
+    lat = sorted(lat)
+    line = {
+        "metric": "C4 mix: 64 uneven scans batched, evals/s", "value": evals / statistics.mean(bat_t),
+        "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp4 (e2m1, fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": "mix64: 64 scans, variants logU[500, 50000], samples logU[200, 5000], "
+                               "case fraction U[0.2, 0.8], top-100 each, %s" %
+                               ("Fisher" if args.ffun else "entropy"),
+                   "scans": len(items), "evals": evals},
+        "sequential_ms": 1000.0 * statistics.mean(seq_t),
+        "batched_ms": 1000.0 * statistics.mean(bat_t),
+        "speedup_vs_sequential": statistics.mean(seq_t) / statistics.mean(bat_t),
+        "results_identical_to_sequential": identical,
+        "rounds": bstats[-1]["rounds"],
+        "scan_latency_in_batch_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1,
+                                                                         int(0.99 * len(lat)))]},
+        "roofline": {"bound": "mfma", "kernel": "k_count_f4_batch",
+                     "achieved": ops / (k_ms / 1000.0) / 1e12, "peak": FP4_MFMA_PEAK_FLOPS / 1e12,
+                     "unit": "TFLOP/s", "frac": ops / (k_ms / 1000.0) / FP4_MFMA_PEAK_FLOPS,
+                     "traffic": None, "count_kernel_ms": k_ms},
+        "cpu_baseline": None, "gen_s": gen_s,
+    }
+    eng.close()
+    for h in panels:
+        h.close()
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -100,6 +178,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "mix64":
+        run_mix(args)
+        return
     import numpy as np
 
     import gepicount_amd as ge

@@ -24,28 +24,39 @@ namespace gepi {
 
 /* One scan's band in a count round. */
 struct BatchDesc {
-    CountArgs g;      /* tile_begin / n_tiles: the band                       */
-    int64_t wg_begin; /* first (XCD-remapped) workgroup of this scan          */
+    CountArgs g; /* tile_begin / n_tiles: the band                            */
+};
+
+/* Workgroup -> (scan, tile).  Workgroups are dispatched to the 8 XCDs round
+ * robin (XCD = blockIdx.x % 8).  Every scan's band is split into 8 contiguous
+ * chunks, one per XCD (so each XCD's L2 sees neighbouring tiles), and each
+ * XCD runs its chunks scan after scan: slot s = blockIdx.x / 8 of XCD x is
+ * in scan i when xprefix[x][i] <= s < xprefix[x][i+1], at tile
+ * tile_begin + xstart[x][i] + (s - xprefix[x][i]).  Chunk sizes follow the
+ * round-robin dealing of the band, so every XCD gets the same number of
+ * workgroups of every scan to within one.                                  */
+struct BatchMap {
+    const int64_t *xprefix; /* [8][n + 1] */
+    const int64_t *xstart;  /* [8][n]     */
+    int32_t n;
 };
 
 template <int FFUN>
 __global__ void __launch_bounds__(F4_THREADS, 2)
-    k_count_f4_batch(const BatchDesc *__restrict__ d, int32_t n_desc, int32_t n_wg) {
-    /* the same XCD-contiguous remap as band_tile, over the whole launch */
-    const int32_t gidx = blockIdx.x;
-    const int32_t xcd = gidx & 7, slot = gidx >> 3;
-    const int32_t per = n_wg >> 3, rem = n_wg & 7;
-    const int64_t lin = (int64_t)xcd * per + (xcd < rem ? xcd : rem) + slot;
-    int32_t lo = 0, hi = n_desc - 1;
+    k_count_f4_batch(const BatchDesc *__restrict__ d, BatchMap map) {
+    const int32_t x = blockIdx.x & 7;
+    const int64_t slot = blockIdx.x >> 3;
+    const int64_t *pre = map.xprefix + (int64_t)x * (map.n + 1);
+    int32_t lo = 0, hi = map.n - 1;
     while (lo < hi) {
         const int32_t mid = (lo + hi + 1) >> 1;
-        if (d[mid].wg_begin <= lin)
+        if (pre[mid] <= slot)
             lo = mid;
         else
             hi = mid - 1;
     }
     const CountArgs g = d[lo].g;
-    count_f4_tile<FFUN>(g, g.tile_begin + (lin - d[lo].wg_begin));
+    count_f4_tile<FFUN>(g, g.tile_begin + map.xstart[(int64_t)x * map.n + lo] + (slot - pre[lo]));
 }
 
 struct ExactDesc {

@@ -67,7 +67,10 @@ void Engine::release() {
     free_.clear();
     for (auto &b : live_) hipFree(b.first);
     live_.clear();
-    for (int i = 0; i < 2; i++) hipFree(d_bdesc_[i]);
+    for (int i = 0; i < 2; i++) {
+        hipFree(d_bdesc_[i]);
+        hipFree(d_map_[i]);
+    }
     hipFree(d_edesc_);
     hipFree(d_tdesc_);
     hipFree(d_rep_);
@@ -116,7 +119,9 @@ int Engine::grow_desc(size_t n) {
     const size_t c = std::max<size_t>(n, 2 * desc_cap_);
     for (int i = 0; i < 2; i++) {
         hipFree(d_bdesc_[i]);
+        hipFree(d_map_[i]);
         d_bdesc_[i] = nullptr;
+        d_map_[i] = nullptr;
     }
     hipFree(d_edesc_);
     hipFree(d_tdesc_);
@@ -125,7 +130,10 @@ int Engine::grow_desc(size_t n) {
     d_tdesc_ = nullptr;
     d_rep_ = nullptr;
     desc_cap_ = 0;
-    for (int i = 0; i < 2; i++) HIP_TRY(hipMalloc(&d_bdesc_[i], sizeof(BatchDesc) * c));
+    for (int i = 0; i < 2; i++) {
+        HIP_TRY(hipMalloc(&d_bdesc_[i], sizeof(BatchDesc) * c));
+        HIP_TRY(hipMalloc(&d_map_[i], sizeof(int64_t) * 8 * (2 * c + 1)));
+    }
     HIP_TRY(hipMalloc(&d_edesc_, sizeof(ExactDesc) * c));
     HIP_TRY(hipMalloc(&d_tdesc_, sizeof(TopkDesc) * c));
     HIP_TRY(hipMalloc(&d_rep_, sizeof(ItemReport) * c));
@@ -140,8 +148,9 @@ size_t Engine::estimate_footprint(const gepi_panel *p, int64_t topmost) {
     const int64_t k = std::min<int64_t>(topmost, 5 * (R * (R + 1) / 2));
     const unsigned long long cap =
         std::max<unsigned long long>((unsigned long long)k + TILE_EVALS_MAX + 1024, 4ull * k);
+    const unsigned long long pcap = std::max<unsigned long long>(cap, 1ull << 20);
     return round_block((size_t)n_k4 * n_var_pad * 3 * 16) + round_block(cap * sizeof(Cand)) +
-           round_block(cap * sizeof(Pend)) + round_block((size_t)4 * p->w64 * sizeof(CompressWord)) +
+           round_block(pcap * sizeof(Pend)) + round_block((size_t)4 * p->w64 * sizeof(CompressWord)) +
            round_block(sizeof(ScanState));
 }
 
@@ -223,7 +232,9 @@ int Engine::add(ScanItem *it) {
     const unsigned long long cap_min = (unsigned long long)it->k + TILE_EVALS_MAX + 1024;
     it->cap = std::max<unsigned long long>(cap_min, 4ull * (unsigned long long)it->k);
     if (getenv("GEPI_CAND_CAP")) it->cap = std::max(env_ull("GEPI_CAND_CAP", 0), cap_min);
-    it->pcap = it->cap;
+    /* a pending list of at least 1 Mi survivors (16 MiB) lets loosely
+     * filtered early bands grow without filling the candidate buffer      */
+    it->pcap = std::max<unsigned long long>(it->cap, 1ull << 20);
     if (getenv("GEPI_PEND_CAP"))
         it->pcap = std::max<unsigned long long>(env_ull("GEPI_PEND_CAP", 0),
                                                 (unsigned long long)TILE_EVALS_MAX + 1024);
@@ -301,28 +312,13 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             it->band_used = std::max<int64_t>(1, (int64_t)((double)it->band_used * f));
     }
     std::vector<BatchDesc> bd[2];
+    std::vector<ScanItem *> grp[2];
     int64_t wg[2] = {0, 0};
     std::vector<ExactDesc> ed(na);
     std::vector<TopkDesc> td(na);
     for (size_t i = 0; i < na; i++) {
         ScanItem *it = active_[i];
-        BatchDesc d;
-        memset(&d, 0, sizeof d);
-        d.g.planes_k = reinterpret_cast<const uint64_t *>(it->d_layout);
-        d.g.tile_begin = it->t0;
-        d.g.n_tiles = (int32_t)it->band_used;
-        d.g.n_var = (int32_t)it->n_var;
-        d.g.n_var_pad = (int32_t)it->n_var_pad;
-        d.g.n_ks = it->n_k4;
-        d.g.ks0 = it->ks0;
-        d.g.st = it->d_state;
-        d.g.pend = it->d_pend;
-        d.g.pcap = it->pcap;
-        d.g.lfact = tabs_.lfact;
-        const int f = it->ffun == GEPI_FFUNC_ENTROPY ? 0 : 1;
-        d.wg_begin = wg[f];
-        wg[f] += it->band_used;
-        bd[f].push_back(d);
+        grp[it->ffun == GEPI_FFUNC_ENTROPY ? 0 : 1].push_back(it);
         ExactDesc &e = ed[i];
         memset(&e, 0, sizeof e);
         e.e.st = it->d_state;
@@ -343,21 +339,72 @@ int Engine::round(std::vector<ScanItem *> &finished) {
         t.k = (uint32_t)it->k;
         t.ffun = it->ffun;
     }
+    /* per objective: descriptors with the longest K loops first (their
+     * workgroups are dispatched first; short ones fill the tail), and the
+     * per-XCD chunk tables of BatchMap */
+    std::vector<int64_t> map[2];
+    for (int f = 0; f < 2; f++) {
+        std::vector<ScanItem *> &g = grp[f];
+        std::stable_sort(g.begin(), g.end(),
+                         [](const ScanItem *a, const ScanItem *b) { return a->n_k4 > b->n_k4; });
+        const size_t n = g.size();
+        map[f].assign(8 * (n + 1) + 8 * n, 0);
+        int64_t *xpre = map[f].data(), *xst = map[f].data() + 8 * (n + 1);
+        int64_t dealt = 0;
+        for (size_t j = 0; j < n; j++) {
+            ScanItem *it = g[j];
+            BatchDesc d;
+            memset(&d, 0, sizeof d);
+            d.g.planes_k = reinterpret_cast<const uint64_t *>(it->d_layout);
+            d.g.tile_begin = it->t0;
+            d.g.n_tiles = (int32_t)it->band_used;
+            d.g.n_var = (int32_t)it->n_var;
+            d.g.n_var_pad = (int32_t)it->n_var_pad;
+            d.g.n_ks = it->n_k4;
+            d.g.ks0 = it->ks0;
+            d.g.st = it->d_state;
+            d.g.pend = it->d_pend;
+            d.g.pcap = it->pcap;
+            d.g.lfact = tabs_.lfact;
+            bd[f].push_back(d);
+            const int64_t b = it->band_used, r = dealt & 7;
+            int64_t start = 0;
+            for (int x = 0; x < 8; x++) {
+                const int64_t chunk = b / 8 + (((x - r) & 7) < (b & 7) ? 1 : 0);
+                xst[(size_t)x * n + j] = start;
+                xpre[(size_t)x * (n + 1) + j + 1] = xpre[(size_t)x * (n + 1) + j] + chunk;
+                start += chunk;
+            }
+            dealt += b;
+        }
+        wg[f] = dealt;
+    }
     for (int f = 0; f < 2; f++)
-        if (!bd[f].empty())
+        if (!bd[f].empty()) {
             HIP_TRY(hipMemcpyAsync(d_bdesc_[f], bd[f].data(), sizeof(BatchDesc) * bd[f].size(),
                                    hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(d_map_[f], map[f].data(), sizeof(int64_t) * map[f].size(),
+                                   hipMemcpyHostToDevice, stream));
+        }
     HIP_TRY(hipMemcpyAsync(d_edesc_, ed.data(), sizeof(ExactDesc) * na, hipMemcpyHostToDevice,
                            stream));
     HIP_TRY(hipMemcpyAsync(d_tdesc_, td.data(), sizeof(TopkDesc) * na, hipMemcpyHostToDevice,
                            stream));
     HIP_TRY(hipEventRecord(ev0_, stream));
-    if (wg[0] > 0)
-        k_count_f4_batch<GEPI_FFUNC_ENTROPY><<<(unsigned)wg[0], F4_THREADS, 0, stream>>>(
-            d_bdesc_[0], (int32_t)bd[0].size(), (int32_t)wg[0]);
-    if (wg[1] > 0)
-        k_count_f4_batch<GEPI_FFUNC_FISHERPV><<<(unsigned)wg[1], F4_THREADS, 0, stream>>>(
-            d_bdesc_[1], (int32_t)bd[1].size(), (int32_t)wg[1]);
+    for (int f = 0; f < 2; f++) {
+        if (wg[f] == 0) continue;
+        const size_t n = bd[f].size();
+        BatchMap m;
+        m.xprefix = d_map_[f];
+        m.xstart = d_map_[f] + 8 * (n + 1);
+        m.n = (int32_t)n;
+        if (f == 0)
+            k_count_f4_batch<GEPI_FFUNC_ENTROPY><<<(unsigned)wg[f], F4_THREADS, 0, stream>>>(
+                d_bdesc_[f], m);
+        else
+            k_count_f4_batch<GEPI_FFUNC_FISHERPV><<<(unsigned)wg[f], F4_THREADS, 0, stream>>>(
+                d_bdesc_[f], m);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev1_, stream));
     k_exact_batch<<<dim3(EXACT_BLOCKS_X, (unsigned)na), 256, 0, stream>>>(d_edesc_);
@@ -395,10 +442,16 @@ int Engine::round(std::vector<ScanItem *> &finished) {
         it->exact_total += r.pend_count;
         const unsigned long long added = r.raw_count - it->count;
         it->count = r.count;
-        const double rate = (double)std::max<unsigned long long>(r.pend_count, added) / (double)used;
-        const int64_t room = (int64_t)(it->cap - (unsigned long long)it->k) / 2;
-        const int64_t next = rate > 0.0 ? (int64_t)((double)room / rate) : it->max_band;
-        it->band = std::max<int64_t>(1, std::min<int64_t>({it->max_band, next, used * 4}));
+        /* next band: survivors filling at most half the pending list and
+         * admissions at most half the candidate room, at this band's rates */
+        const double pend_rate = (double)r.pend_count / (double)used;
+        const double adm_rate = (double)added / (double)used;
+        int64_t next = it->max_band;
+        if (pend_rate > 0.0) next = std::min<int64_t>(next, (int64_t)((double)(it->pcap / 2) / pend_rate));
+        if (adm_rate > 0.0)
+            next = std::min<int64_t>(
+                next, (int64_t)((double)((it->cap - (unsigned long long)it->k) / 2) / adm_rate));
+        it->band = std::max<int64_t>(1, std::min<int64_t>(next, used * 4));
         if (it->forced_band) it->band = std::min<int64_t>(it->forced_band, used * 4);
         it->t0 += used;
         if (it->t0 >= it->n_tiles) {

@@ -91,6 +91,7 @@ class Engine {
     std::map<void *, size_t> live_;
     size_t in_use_ = 0;
     gepi::BatchDesc *d_bdesc_[2] = {nullptr, nullptr};
+    int64_t *d_map_[2] = {nullptr, nullptr};
     gepi::ExactDesc *d_edesc_ = nullptr;
     gepi::TopkDesc *d_tdesc_ = nullptr;
     gepi::ItemReport *d_rep_ = nullptr;

@@ -10,6 +10,10 @@ which is the first .bim allele (A1): 2 copies -> code 0 (hom A1), 1 -> code 2
           5 planted XOR pairs with case/control odds ratio 2.0
   large   100,000 x 8,192 (2,458 cases / 5,734 controls), MAF ~ Beta(0.5, 2)
           clipped to [0.01, 0.5], 2% missing
+
+and the mix of uneven scans (mix_specs): variants ~ logU[500, 50,000],
+samples ~ logU[200, 5,000], case fraction ~ U[0.2, 0.8], MAF ~ U[0.01, 0.5],
+1% missing, top-100 each.
 """
 import os
 
@@ -75,10 +79,32 @@ def _plant_xor(rng, codes, maf, cls, a, b, odds=2.0):
         codes[v] = c
 
 
-def make_panel(workload, n_var=None, seed=None, chunk=4096):
+def mix_specs(n_scans=64, seed=1004):
+    """Specs of the uneven-scan mix: list of dicts usable as make_panel(spec=...)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_scans):
+        V = int(round(np.exp(rng.uniform(np.log(500), np.log(50000)))))
+        n = int(round(np.exp(rng.uniform(np.log(200), np.log(5000)))))
+        n_case = int(round(rng.uniform(0.2, 0.8) * n))
+        out.append(dict(n_var=V, n_samples=n, n_case=n_case, maf=("uniform", 0.01, 0.5),
+                        miss=0.01, planted=0, seed=seed * 1000 + i, topk=100))
+    return out
+
+
+def pheno_for(n, n_case, seed):
+    """A seeded phenotype vector with n_case cases (2) among n samples (1)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cls = np.zeros(n, np.int64)
+    cls[rng.permutation(n)[:n_case]] = 1
+    return np.where(cls == 1, 2, 1)
+
+
+def make_panel(workload=None, n_var=None, seed=None, chunk=4096, spec=None):
     """Returns dict(payload [V][ceil(n/4)] uint8, pheno [n] int, n_var, n_samples,
-    planted [(a, b)], maf)."""
-    spec = dict(WORKLOADS[workload])
+    planted [(a, b)], maf).  `spec` (a dict like WORKLOADS' entries) replaces
+    the named workload."""
+    spec = dict(WORKLOADS[workload] if spec is None else spec)
     if n_var is not None:
         spec["n_var"] = int(n_var)
     rng = np.random.Generator(np.random.PCG64(spec["seed"] if seed is None else seed))
