@@ -4,10 +4,10 @@ The compute lives in libgepicount.so (include/gepicount.h, gepicount_amd/csrc/);
 this package is its ctypes binding plus a synthetic PLINK panel generator.
 """
 from .api import (GEPI_FFUNC_ENTROPY, GEPI_FFUNC_FISHERPV, FUNC_NAMES, RESULT_DTYPE, Engine,
-                  GepiError, Panel, PlinkData, device_count, fisher_pval, lib, process_gpu_1,
-                  read_plink)
+                  GepiError, Panel, PlinkData, Server, device_count, fisher_pval, lib,
+                  process_gpu_1, read_plink)
 
 __all__ = ["GEPI_FFUNC_ENTROPY", "GEPI_FFUNC_FISHERPV", "FUNC_NAMES", "RESULT_DTYPE", "Engine",
-           "GepiError",
+           "GepiError", "Server",
            "Panel", "PlinkData", "device_count", "fisher_pval", "lib", "process_gpu_1",
            "read_plink"]

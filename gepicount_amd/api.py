@@ -30,7 +30,10 @@ EXPORTS = ("gepi_params_init", "gepi_process_gpu_1", "gepi_panel_create", "gepi_
            "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_plink_open",
            "gepi_plink_close", "gepi_plink_check_quality", "gepi_print_interactions",
            "gepi_fisher_pval", "gepi_last_error", "gepi_version", "gepi_device_count",
-           "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch")
+           "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch",
+           "gepi_server_create", "gepi_server_destroy", "gepi_server_add_panel_bed",
+           "gepi_server_release_panel", "gepi_server_submit", "gepi_server_done",
+           "gepi_server_wait", "gepi_server_get_stats")
 
 
 class GepiError(RuntimeError):
@@ -70,6 +73,22 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("ms_batch", ctypes.c_double), ("ms_prepare", ctypes.c_double),
                 ("ms_count_kernels", ctypes.c_double), ("evals", ctypes.c_int64),
                 ("rounds", ctypes.c_int32), ("items_failed", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerParams(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("hbm_budget", ctypes.c_int64), ("round_wg_budget", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("failed", ctypes.c_int64), ("evals", ctypes.c_int64), ("rounds", ctypes.c_int64),
+                ("max_active", ctypes.c_int32), ("panels", ctypes.c_int32),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("busy_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -136,11 +155,22 @@ def lib():
     l.gepi_engine_destroy.argtypes = [vp]
     l.gepi_engine_scan_batch.argtypes = [vp, ctypes.POINTER(BatchItem), ctypes.c_int32,
                                          ctypes.POINTER(BatchStats)]
-    for name in ("gepi_process_gpu_1", "gepi_panel_create", "gepi_panel_create_bed",
-                 "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_plink_open",
-                 "gepi_plink_check_quality", "gepi_print_interactions", "gepi_device_count",
-                 "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch"):
-        getattr(l, name).restype = ctypes.c_int
+    u64 = ctypes.c_uint64
+    l.gepi_server_create.argtypes = [ctypes.POINTER(ServerParams), ctypes.POINTER(vp)]
+    l.gepi_server_destroy.argtypes = [vp]
+    l.gepi_server_add_panel_bed.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int32,
+                                            ctypes.POINTER(u64)]
+    l.gepi_server_release_panel.argtypes = [vp, u64]
+    l.gepi_server_submit.argtypes = [vp, u64, u64p, u64p, ctypes.c_int64, ctypes.c_int32,
+                                     ctypes.POINTER(u64)]
+    l.gepi_server_done.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int32)]
+    l.gepi_server_wait.argtypes = [vp, u64, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ScanStats)]
+    l.gepi_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    for name in EXPORTS:
+        if name not in ("gepi_params_init", "gepi_plink_close", "gepi_fisher_pval",
+                        "gepi_last_error", "gepi_version"):
+            getattr(l, name).restype = ctypes.c_int
     _lib = l
     return l
 
@@ -308,6 +338,77 @@ class Engine:
     def close(self):
         if self._h:
             _check(lib().gepi_engine_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop (gepi_server_*): resident panels keyed by content hash,
+    requests admitted between rounds of the batched engine by the server's
+    worker thread.  Thread safe."""
+
+    def __init__(self, device=0, hbm_budget=0, round_wg_budget=0):
+        par = ServerParams(int(device), 0, int(hbm_budget), int(round_wg_budget))
+        h = ctypes.c_void_p()
+        _check(lib().gepi_server_create(ctypes.byref(par), ctypes.byref(h)))
+        self._h = h
+        self._words = {}
+
+    def add_panel_bed(self, bed_payload, n_variants, n_samples):
+        payload = np.ascontiguousarray(bed_payload, np.uint8)
+        buf = payload if payload.size else np.zeros(1, np.uint8)
+        pid = ctypes.c_uint64()
+        _check(lib().gepi_server_add_panel_bed(self._h, buf.ctypes.data, int(n_variants),
+                                               int(n_samples), ctypes.byref(pid)))
+        self._words[pid.value] = ((int(n_samples) + 63) // 64, 2 * int(n_variants))
+        return pid.value
+
+    def release_panel(self, panel_id):
+        _check(lib().gepi_server_release_panel(self._h, ctypes.c_uint64(panel_id)))
+
+    def submit(self, panel_id, mask_ctrl, mask_case, topmost=1000, ffun=GEPI_FFUNC_ENTROPY):
+        # an unknown id is the library's to reject (GEPI_ERR_ARG)
+        words, rows = self._words.get(panel_id, (1, 0))
+        m0, m1 = _masks(mask_ctrl, mask_case, words)
+        t = ctypes.c_uint64()
+        _check(lib().gepi_server_submit(self._h, ctypes.c_uint64(panel_id), _u64p(m0), _u64p(m1),
+                                        int(topmost), int(ffun), ctypes.byref(t)))
+        return (t.value, int(topmost), rows)
+
+    def done(self, ticket):
+        d = ctypes.c_int32(0)
+        _check(lib().gepi_server_done(self._h, ctypes.c_uint64(ticket[0]), ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        """Blocks until the request is done; returns (results, stats dict, status)."""
+        t, topmost, rows = ticket
+        out = _result_buffer(topmost, rows)
+        n = ctypes.c_int64(0)
+        st = ScanStats()
+        rc = lib().gepi_server_wait(self._h, ctypes.c_uint64(t), out.ctypes.data, len(out),
+                                    ctypes.byref(n), ctypes.byref(st))
+        return out[:n.value].copy(), st.as_dict(), int(rc)
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().gepi_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            _check(lib().gepi_server_destroy(self._h))
             self._h = None
 
     def __enter__(self):

@@ -59,26 +59,38 @@ int Engine::init(int dev) {
 }
 
 void Engine::release() {
-    hipSetDevice(device);
-    if (stream) hipStreamSynchronize(stream);
+    /* idempotent: every handle is cleared after it is released */
+    if (stream) {
+        hipSetDevice(device);
+        hipStreamSynchronize(stream);
+    }
     for (ScanItem *it : active_) drop(it, GEPI_ERR_INTERNAL);
     active_.clear();
     for (auto &b : free_) hipFree(b.second);
     free_.clear();
     for (auto &b : live_) hipFree(b.first);
     live_.clear();
+    in_use_ = 0;
     for (int i = 0; i < 2; i++) {
         hipFree(d_bdesc_[i]);
         hipFree(d_map_[i]);
+        d_bdesc_[i] = nullptr;
+        d_map_[i] = nullptr;
     }
     hipFree(d_edesc_);
     hipFree(d_tdesc_);
     hipFree(d_rep_);
+    d_edesc_ = nullptr;
+    d_tdesc_ = nullptr;
+    d_rep_ = nullptr;
+    desc_cap_ = 0;
     hipFree(tabs_.lhi);
     hipFree(tabs_.llo);
     hipFree(tabs_.lfact);
+    tabs_ = DeviceTables();
     if (ev0_) hipEventDestroy(ev0_);
     if (ev1_) hipEventDestroy(ev1_);
+    ev0_ = ev1_ = nullptr;
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
 }

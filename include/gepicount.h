@@ -172,6 +172,59 @@ int gepi_engine_destroy(gepi_engine *engine);
 int gepi_engine_scan_batch(gepi_engine *engine, gepi_batch_item *items, int32_t n_items,
                            gepi_batch_stats *stats);
 
+/* ---- serving loop -----------------------------------------------------------
+ * A server owns an engine and a worker thread.  Requests (a resident panel
+ * and class masks) are admitted between rounds while the device memory they
+ * need fits the budget (continuous batching: a request submitted while
+ * others run joins them at the next round).  Panels are resident and keyed
+ * by a 128-bit content hash of their genotype bytes: adding the same bytes
+ * again returns the same id.  A re-scan with a new phenotype rebuilds only
+ * the class layout, on the device.  Each result is identical to gepi_scan's
+ * (fp4 path) on the same input.  No counterpart in the reference (one scan
+ * per process, src/gEpiCount.cpp:84-87).  All functions are thread safe.   */
+typedef struct gepi_server gepi_server;
+
+typedef struct gepi_server_params {
+    int32_t device;
+    int32_t reserved;
+    int64_t hbm_budget;      /* scan workspace admitted at once, bytes; 0 = 3/4
+                                of the device memory free at creation; a
+                                request larger than the budget runs alone      */
+    int64_t round_wg_budget; /* workgroups per count round; 0 = 262,144: bounds
+                                how long a new request waits to join          */
+} gepi_server_params;
+
+typedef struct gepi_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t evals;           /* of completed requests                          */
+    int64_t rounds;
+    int32_t max_active;      /* most requests in one round                     */
+    int32_t panels;
+    double p50_ms, p99_ms, mean_ms; /* submit -> completion                    */
+    double busy_ms;          /* time spent in rounds                           */
+} gepi_server_stats;
+
+int gepi_server_create(const gepi_server_params *par, gepi_server **out);
+/* Waits for the requests in flight, then releases everything. */
+int gepi_server_destroy(gepi_server *server);
+/* Resident panel from SNP-major .bed bytes (as gepi_panel_create_bed); the
+ * id is the content hash. */
+int gepi_server_add_panel_bed(gepi_server *server, const uint8_t *bed_payload, int64_t n_variants,
+                              int32_t n_samples, uint64_t *panel_id);
+/* Drops one reference; the panel goes when no request uses it. */
+int gepi_server_release_panel(gepi_server *server, uint64_t panel_id);
+/* Queues a scan; masks are copied.  *ticket identifies it for _wait. */
+int gepi_server_submit(gepi_server *server, uint64_t panel_id, const uint64_t *mask_ctrl,
+                       const uint64_t *mask_case, int64_t topmost, int32_t ffun,
+                       uint64_t *ticket);
+/* Reports whether a ticket is done (1) or not (0) without blocking. */
+int gepi_server_done(gepi_server *server, uint64_t ticket, int32_t *done);
+/* Blocks until the request is done; copies min(capacity, n) results and
+ * returns the request's status.  A ticket can be waited for once. */
+int gepi_server_wait(gepi_server *server, uint64_t ticket, gepi_result *tops, int64_t capacity,
+                     int64_t *n_tops, gepi_scan_stats *stats);
+int gepi_server_get_stats(gepi_server *server, gepi_server_stats *out);
+
 /* ---- host-side PLINK reader and report ---------------------------------- */
 /* Replaces BinPlinkReader<ui8> (src/reader/BinPlinkReader.h, PlinkReader.h).
  * Fields are read-only for the caller; release with gepi_plink_close.        */

@@ -30,6 +30,19 @@ def test_header_matches_binding_list():
     assert declared_functions() == sorted(api.EXPORTS)
 
 
+def test_binding_declares_every_signature():
+    """Every entry point with parameters has ctypes argtypes: without them
+    ctypes passes Python ints as 32-bit C ints and truncates addresses."""
+    lib = ge.lib()
+    for name in api.EXPORTS:
+        fn = getattr(lib, name)
+        if name in ("gepi_last_error", "gepi_version"):
+            continue
+        assert fn.argtypes is not None, name
+        if name not in ("gepi_params_init", "gepi_plink_close"):  # void
+            assert fn.restype is not None, name
+
+
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(api.LIB_PATH)
     for name in declared_functions():
@@ -128,6 +141,12 @@ def test_scan_fails_loudly_without_gpu():
     bits = np.zeros((16, 4), np.uint64)
     with pytest.raises(ge.GepiError) as e:
         ge.process_gpu_1(bits, bits, None, None, 256, topmost=10)
+    assert e.value.code == -5
+    with pytest.raises(ge.GepiError) as e:
+        ge.Engine()
+    assert e.value.code == -5
+    with pytest.raises(ge.GepiError) as e:
+        ge.Server()
     assert e.value.code == -5
 
 
