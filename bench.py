@@ -38,8 +38,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="large", choices=["small", "medium", "large", "mix64"],
-                    help="mix64: 64 uneven scans, batched (gepi_engine) against sequential")
+    ap.add_argument("--workload", default="large",
+                    choices=["small", "medium", "large", "mix64", "serve"],
+                    help="mix64: 64 uneven scans, batched (gepi_engine) against sequential; "
+                         "serve: Poisson arrivals on the serving loop (gepi_server)")
+    ap.add_argument("--qps", type=float, default=40.0, help="serve: arrival rate")
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the run")
     ap.add_argument("--n-var", type=int, default=None, help="override variant count")
     ap.add_argument("--topk", type=int, default=10000)
     ap.add_argument("--ffun", type=int, default=0)
@@ -169,6 +173,100 @@ def run_mix(args):
     print(json.dumps(line))
 
 
+def run_serve(args):
+    """Serving loop under Poisson arrivals at a fixed rate: 80% of the requests
+    are scans of the uneven mix (its panels and phenotypes, top-100), 20%
+    re-scan a resident 50,000 x 4,000 panel with a new phenotype vector
+    (case fraction U[0.2, 0.8], top-100).  Panels are resident before the
+    run; latency is submit -> completion as the server measures it."""
+    import threading
+
+    import numpy as np
+
+    import gepicount_amd as ge
+    from gepicount_amd import synth
+
+    t = time.perf_counter()
+    specs = synth.mix_specs()
+    big_spec = dict(n_var=50000, n_samples=4000, n_case=2000, maf=("uniform", 0.01, 0.5),
+                    miss=0.01, planted=0, seed=1005)
+    srv = ge.Server()
+    mix = []
+    for sp in specs:
+        p = synth.make_panel(spec=sp)
+        pid = srv.add_panel_bed(p["payload"], p["n_var"], p["n_samples"])
+        m0, m1 = synth.masks_from_pheno(p["pheno"])
+        mix.append((pid, m0, m1, p["n_var"], p))
+    big = synth.make_panel(spec=big_spec)
+    big_id = srv.add_panel_bed(big["payload"], big["n_var"], big["n_samples"])
+    gen_s = time.perf_counter() - t
+    rng = np.random.Generator(np.random.PCG64(1006))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, size=args.requests))
+    reqs = []
+    for i in range(args.requests):
+        if rng.random() < 0.2:
+            n = big_spec["n_samples"]
+            pheno = synth.pheno_for(n, int(round(rng.uniform(0.2, 0.8) * n)), 2000 + i)
+            m0, m1 = synth.masks_from_pheno(pheno)
+            reqs.append((big_id, m0, m1, big_spec["n_var"], "rescan", None))
+        else:
+            j = int(rng.integers(0, len(mix)))
+            pid, m0, m1, nv, p = mix[j]
+            reqs.append((pid, m0, m1, nv, "mix", p))
+    # warm the server (first launches, allocator) with a few requests
+    for pid, m0, m1, _, _, _ in reqs[:4]:
+        srv.wait(srv.submit(pid, m0, m1, topmost=100, ffun=args.ffun))
+    base = srv.stats()
+    tickets = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, (pid, m0, m1, _, _, _) in enumerate(reqs):
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(pid, m0, m1, topmost=100, ffun=args.ffun)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    results = [None] * len(reqs)
+    for i in range(len(reqs)):
+        while tickets[i] is None:
+            time.sleep(0.0005)
+        results[i] = srv.wait(tickets[i])
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    lat = sorted(r[1]["ms_scan"] for r in results)
+    evals = sum(r[1]["evals"] for r in results)
+    # a sample of responses against the same scan run alone
+    identical = True
+    for i in [k for k, r in enumerate(reqs) if r[4] == "mix"][:4]:
+        p = reqs[i][5]
+        with ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"]) as h:
+            alone, _ = h.scan(reqs[i][1], reqs[i][2], topmost=100, ffun=args.ffun, count_path=3)
+        identical &= bool(np.array_equal(alone, results[i][0]))
+    srv.close()
+    line = {
+        "metric": "C5 serving: evals/s under Poisson arrivals; p50/p99 request latency",
+        "value": evals / wall, "unit": "evals/s", "n_gpus": 1, "steps": len(reqs), "warmup": 4,
+        "ms_per_step": 1000.0 * wall / len(reqs), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp4 (e2m1, fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": "serve: Poisson %.0f QPS, 80%% scans of the uneven 64-scan mix, "
+                               "20%% re-scans of a resident 50000 x 4000 panel, top-100, %s"
+                               % (args.qps, "Fisher" if args.ffun else "entropy"),
+                   "requests": len(reqs), "qps": args.qps},
+        "latency_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))],
+                       "mean": sum(lat) / len(lat), "max": lat[-1]},
+        "achieved_qps": len(reqs) / wall,
+        "gpu_busy_frac": (st["busy_ms"] - base["busy_ms"]) / (1000.0 * wall),
+        "rounds": st["rounds"] - base["rounds"], "max_active": st["max_active"],
+        "failed": st["failed"], "responses_identical_to_alone": identical,
+        "roofline": None, "cpu_baseline": None, "gen_s": gen_s,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -180,6 +278,9 @@ def main():
         sys.exit(2)
     if args.workload == "mix64":
         run_mix(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
     import numpy as np
 
