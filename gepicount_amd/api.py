@@ -27,7 +27,8 @@ RESULT_DTYPE = np.dtype([("sA", "<i8"), ("sB", "<i8"), ("w", "<i4"), ("x", "<i4"
 
 # Names of every entry point declared in include/gepicount.h.
 EXPORTS = ("gepi_params_init", "gepi_process_gpu_1", "gepi_panel_create", "gepi_panel_create_bed",
-           "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_plink_open",
+           "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_panel_qa",
+           "gepi_panel_filter", "gepi_plink_open",
            "gepi_plink_close", "gepi_plink_check_quality", "gepi_print_interactions",
            "gepi_fisher_pval", "gepi_last_error", "gepi_version", "gepi_device_count",
            "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch",
@@ -138,6 +139,10 @@ def lib():
     l.gepi_scan.argtypes = [vp, u64p, u64p, ctypes.POINTER(Params), vp,
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ScanStats)]
     l.gepi_panel_download.argtypes = [vp, u64p, u64p]
+    i32p, i64p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)
+    l.gepi_panel_qa.argtypes = [vp, ctypes.c_float, ctypes.c_float, i32p, i32p, i64p, i64p]
+    l.gepi_panel_filter.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.POINTER(vp), i64p,
+                                    i64p, u64p]
     l.gepi_plink_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(PlinkStruct))]
     l.gepi_plink_close.argtypes = [ctypes.POINTER(PlinkStruct)]
     l.gepi_plink_close.restype = None
@@ -274,6 +279,32 @@ class Panel:
         _check(lib().gepi_scan(self._h, _u64p(m0), _u64p(m1), ctypes.byref(par),
                                out.ctypes.data, ctypes.byref(n), ctypes.byref(st)))
         return out[:n.value].copy(), st.as_dict()
+
+    def qa(self, max_miss_snps=0.05, max_miss_samples=0.05):
+        """Missing calls per variant and per sample, and how many exceed the
+        thresholds (gepi_panel_qa)."""
+        n_var = self.n_rows // 2
+        vm = np.zeros(max(n_var, 1), np.int32)
+        sm = np.zeros(max(self.n_samples, 1), np.int32)
+        bv, bs = ctypes.c_int64(0), ctypes.c_int64(0)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        _check(lib().gepi_panel_qa(self._h, float(max_miss_snps), float(max_miss_samples),
+                                   vm.ctypes.data_as(i32p), sm.ctypes.data_as(i32p),
+                                   ctypes.byref(bv), ctypes.byref(bs)))
+        return vm[:n_var], sm[:self.n_samples], bv.value, bs.value
+
+    def filter(self, max_miss_snps=0.05, max_miss_samples=0.05):
+        """checkQuality(remove=true): (new Panel of the conforming variants,
+        their original indices, mask of conforming samples)."""
+        n_var = self.n_rows // 2
+        kept = np.zeros(max(n_var, 1), np.int64)
+        nk = ctypes.c_int64(0)
+        keep = np.zeros(max(self.words, 1), np.uint64)
+        h = ctypes.c_void_p()
+        _check(lib().gepi_panel_filter(self._h, float(max_miss_snps), float(max_miss_samples),
+                                       ctypes.byref(h), kept.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       ctypes.byref(nk), _u64p(keep)))
+        return Panel(h, 2 * nk.value, self.n_samples), kept[:nk.value].copy(), keep
 
     def download(self):
         bits = np.zeros((self.n_rows, max(self.words, 1)), np.uint64)

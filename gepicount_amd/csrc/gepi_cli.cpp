@@ -38,6 +38,8 @@ void print_help() {
             "        --count-path      P    popcount | mfma | fp4 | auto  [auto]\n"
             "        --miss-snps       F    Max missing values fraction for SNPs [0.05]\n"
             "        --miss-samples    F    Max missing values fraction for samples  [0.05]\n"
+            "        --qa-remove            Drop SNPs and samples beyond those fractions before\n"
+            "                               scanning (QA on the GPU) [no]\n"
             "   -g | --gold                 Not available: the CPU gold scan is oracle/gepicount_gold\n\n",
             PROGNAME, GEPI_DEFAULT_TOPMOST);
 }
@@ -52,7 +54,7 @@ int main(int argc, char **argv) {
     gepi_params_init(&par);
     std::string root;
     float miss_snps = 0.05f, miss_samp = 0.05f;
-    bool ok = true, diag = false;
+    bool ok = true, diag = false, qa_remove = false;
     for (int j = 1; j < argc; ++j) {
         std::string a(argv[j]);
         if (a == "-g" || a == "--gold") {
@@ -89,6 +91,8 @@ int main(int argc, char **argv) {
             miss_snps = (float)atof(argv[++j]);
         } else if (a == "--miss-samples" && need(j, argc, 1)) {
             miss_samp = (float)atof(argv[++j]);
+        } else if (a == "--qa-remove") {
+            qa_remove = true;
         } else {
             fprintf(stderr, "Unknown option: %s\n", a.c_str());
             ok = false;
@@ -137,7 +141,33 @@ int main(int argc, char **argv) {
     gepi_panel *panel = nullptr;
     int rc = gepi_panel_create_bed(pr->bed_payload, pr->n_rows / 2, pr->n_samples, par.gpu_num,
                                    &panel);
-    if (rc == GEPI_OK) {
+    if (rc == GEPI_OK && qa_remove) {
+        /* checkQuality(remove = true) on the device: scan the conforming
+         * variants over the conforming samples, then report original rows */
+        gepi_panel *kept = nullptr;
+        std::vector<int64_t> idx((size_t)(pr->n_rows / 2 > 0 ? pr->n_rows / 2 : 1));
+        std::vector<uint64_t> keep((size_t)(pr->words > 0 ? pr->words : 1));
+        int64_t nk = 0;
+        rc = gepi_panel_filter(panel, miss_snps, miss_samp, &kept, idx.data(), &nk, keep.data());
+        gepi_panel_destroy(panel);
+        panel = nullptr;
+        if (rc == GEPI_OK) {
+            std::vector<uint64_t> m0((size_t)pr->words), m1((size_t)pr->words);
+            for (int32_t w = 0; w < pr->words; w++) {
+                m0[(size_t)w] = pr->mask_ctrl[w] & keep[(size_t)w];
+                m1[(size_t)w] = pr->mask_case[w] & keep[(size_t)w];
+            }
+            printf("QA removal: %lld of %lld SNPs kept\n", (long long)nk,
+                   (long long)(pr->n_rows / 2));
+            rc = gepi_scan(kept, m0.data(), m1.data(), &par, tops.data(), &n, &st);
+            gepi_panel_destroy(kept);
+            for (int64_t i = 0; i < n && rc == GEPI_OK; i++) {
+                gepi_result &r = tops[(size_t)i];
+                r.sA = 2 * idx[(size_t)(r.sA / 2)] + (r.sA & 1);
+                r.sB = 2 * idx[(size_t)(r.sB / 2)] + (r.sB & 1);
+            }
+        }
+    } else if (rc == GEPI_OK) {
         rc = gepi_scan(panel, pr->mask_ctrl, pr->mask_case, &par, tops.data(), &n, &st);
         gepi_panel_destroy(panel);
     }

@@ -638,6 +638,125 @@ extern "C" int gepi_panel_destroy(gepi_panel *panel) {
     return GEPI_OK;
 }
 
+namespace {
+
+/* Missing-call counts of a resident panel: per variant (n_var) and per
+ * sample (n_samples), on the panel's device and stream. */
+int panel_missing(gepi_panel *p, std::vector<int32_t> &var_miss, std::vector<int32_t> &samp_miss) {
+    var_miss.assign((size_t)p->n_var, 0);
+    samp_miss.assign((size_t)p->n_samples, 0);
+    if (p->n_var == 0 || p->n_samples == 0) {
+        for (auto &m : var_miss) m = p->n_samples;
+        for (auto &m : samp_miss) m = (int32_t)p->n_var;
+        return GEPI_OK;
+    }
+    int32_t *d_vm = nullptr, *d_sv = nullptr;
+    HIP_TRY(hipMalloc(&d_vm, sizeof(int32_t) * (size_t)p->n_var));
+    if (hipMalloc(&d_sv, sizeof(int32_t) * (size_t)p->n_samples) != hipSuccess) {
+        hipFree(d_vm);
+        return gepi_fail(GEPI_ERR_NOMEM, "QA buffers");
+    }
+    hipMemsetAsync(d_sv, 0, sizeof(int32_t) * (size_t)p->n_samples, p->stream);
+    k_qa_variants<<<(unsigned)((p->n_var + 3) / 4), 256, 0, p->stream>>>(p->d_nat, p->n_var, p->w64,
+                                                                        p->n_samples, d_vm);
+    const int32_t chunk = 1024;
+    dim3 grid((unsigned)((p->n_samples + 255) / 256), (unsigned)((p->n_var + chunk - 1) / chunk));
+    k_qa_samples<<<grid, 256, 0, p->stream>>>(p->d_nat, p->n_var, p->w64, p->n_samples, chunk, d_sv);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(var_miss.data(), d_vm, sizeof(int32_t) * (size_t)p->n_var,
+                           hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(samp_miss.data(), d_sv, sizeof(int32_t) * (size_t)p->n_samples,
+                           hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    hipFree(d_vm);
+    hipFree(d_sv);
+    if (e != hipSuccess)
+        return gepi_fail(GEPI_ERR_DEVICE, "QA kernels failed: %s", hipGetErrorString(e));
+    for (auto &m : samp_miss) m = (int32_t)p->n_var - m; /* valid calls -> missing */
+    return GEPI_OK;
+}
+
+/* The reference's tests (PlinkReader.h:285, :316) on row counts: a row
+ * fraction equals the variant fraction since both alleles share calls. */
+inline bool variant_bad(int32_t miss, int32_t n_samples, float thr) {
+    return n_samples > 0 && (float)miss / (float)n_samples > thr;
+}
+inline bool sample_bad(int32_t miss, int64_t n_var, float thr) {
+    return n_var > 0 && (float)(2 * (int64_t)miss) / (float)(2 * n_var) > thr;
+}
+
+} // namespace
+
+extern "C" int gepi_panel_qa(gepi_panel *p, float max_miss_snps, float max_miss_samples,
+                             int32_t *variant_missing, int32_t *sample_missing,
+                             int64_t *n_bad_variants, int64_t *n_bad_samples) {
+    if (!p) return gepi_fail(GEPI_ERR_ARG, "null panel");
+    int rc;
+    if ((rc = set_device(p->device)) != GEPI_OK) return rc;
+    std::vector<int32_t> vm, sm;
+    if ((rc = panel_missing(p, vm, sm)) != GEPI_OK) return rc;
+    int64_t bv = 0, bs = 0;
+    for (int32_t m : vm) bv += variant_bad(m, p->n_samples, max_miss_snps) ? 1 : 0;
+    for (int32_t m : sm) bs += sample_bad(m, p->n_var, max_miss_samples) ? 1 : 0;
+    if (variant_missing && !vm.empty()) memcpy(variant_missing, vm.data(), sizeof(int32_t) * vm.size());
+    if (sample_missing && !sm.empty()) memcpy(sample_missing, sm.data(), sizeof(int32_t) * sm.size());
+    if (n_bad_variants) *n_bad_variants = bv;
+    if (n_bad_samples) *n_bad_samples = bs;
+    return GEPI_OK;
+}
+
+extern "C" int gepi_panel_filter(gepi_panel *p, float max_miss_snps, float max_miss_samples,
+                                 gepi_panel **out, int64_t *kept_variants, int64_t *n_kept,
+                                 uint64_t *keep_samples) {
+    if (!p || !out) return gepi_fail(GEPI_ERR_ARG, "null argument");
+    *out = nullptr;
+    int rc;
+    if ((rc = set_device(p->device)) != GEPI_OK) return rc;
+    std::vector<int32_t> vm, sm;
+    if ((rc = panel_missing(p, vm, sm)) != GEPI_OK) return rc;
+    std::vector<int64_t> keep;
+    for (int64_t v = 0; v < p->n_var; v++)
+        if (!variant_bad(vm[(size_t)v], p->n_samples, max_miss_snps)) keep.push_back(v);
+    if (keep_samples) {
+        memset(keep_samples, 0, sizeof(uint64_t) * (size_t)p->w64);
+        for (int32_t s = 0; s < p->n_samples; s++)
+            if (!sample_bad(sm[(size_t)s], p->n_var, max_miss_samples))
+                keep_samples[s >> 6] |= 1ull << (s & 63);
+    }
+    gepi_panel *q = new gepi_panel();
+    if ((rc = panel_alloc(q, p->device, (int64_t)keep.size(), p->n_samples)) != GEPI_OK) {
+        panel_free(q);
+        return rc;
+    }
+    if (!keep.empty() && p->w64 > 0) {
+        int64_t *d_idx = nullptr;
+        if (hipMalloc(&d_idx, sizeof(int64_t) * keep.size()) != hipSuccess) {
+            panel_free(q);
+            return gepi_fail(GEPI_ERR_NOMEM, "filter index");
+        }
+        hipMemcpyAsync(d_idx, keep.data(), sizeof(int64_t) * keep.size(), hipMemcpyHostToDevice,
+                       q->stream);
+        const int64_t threads = (int64_t)keep.size() * 3 * p->w64;
+        /* the source panel's pending work (its own stream) is complete: the
+         * QA above synchronised it */
+        k_gather_variants<<<(unsigned)((threads + 255) / 256), 256, 0, q->stream>>>(
+            p->d_nat, d_idx, (int64_t)keep.size(), p->w64, q->d_nat);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(q->stream);
+        hipFree(d_idx);
+        if (e != hipSuccess) {
+            panel_free(q);
+            return gepi_fail(GEPI_ERR_DEVICE, "variant gather failed: %s", hipGetErrorString(e));
+        }
+    }
+    if (kept_variants && !keep.empty()) memcpy(kept_variants, keep.data(), sizeof(int64_t) * keep.size());
+    if (n_kept) *n_kept = (int64_t)keep.size();
+    *out = q;
+    return GEPI_OK;
+}
+
 extern "C" int gepi_panel_download(gepi_panel *p, uint64_t *bit_data, uint64_t *valid_data) {
     if (!p || !bit_data || !valid_data) return gepi_fail(GEPI_ERR_ARG, "null argument");
     int rc;

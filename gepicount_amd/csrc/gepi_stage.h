@@ -13,10 +13,16 @@
  *                  64-sample words, so that one K step of a tile is contiguous.
  * k_transpose_f4   popcount layout -> fp4 MFMA layout [n_k4][n_var_pad*3] of
  *                  128-sample rows (uint4), one K step of a tile contiguous.
+ * k_qa_variants    missing calls per variant (one wave per variant)
+ * k_qa_samples     missing calls per sample (a workgroup per 256 samples and
+ *                  a chunk of variants; partial counts added atomically)
+ * k_gather_variants  copies the planes of selected variants to a new panel
  * k_class_f4       panel -> fp4 MFMA layout directly: per plane row, the
  *                  samples of each class are compressed (stable) out of the
  *                  file order with mask-driven bit compression, and streamed
  *                  out in 128-sample rows (class 0 steps, then class 1).
+ * The QA kernels replace the loops of checkQuality (src/reader/
+ * PlinkReader.h:237-335), which count missing calls on the host.
  */
 #ifndef GEPI_STAGE_H
 #define GEPI_STAGE_H
@@ -256,6 +262,53 @@ static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restr
     }
     s0.finish(out, rows, r);
     s1.finish(out, rows, r);
+}
+
+/* missing[v] = n_samples - popc(valid plane of variant v); 64 threads per
+ * variant.  Samples beyond n_samples are never valid in a panel. */
+static __global__ void __launch_bounds__(256) k_qa_variants(const uint64_t *__restrict__ nat,
+                                                            int64_t n_var, int32_t w64,
+                                                            int32_t n_samples,
+                                                            int32_t *__restrict__ missing) {
+    const int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (v >= n_var) return;
+    const uint64_t *valid = nat + (v * 3 + 2) * w64;
+    int32_t c = 0;
+    for (int32_t w = lane; w < w64; w += 64) c += __popcll(valid[w]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) missing[v] = n_samples - c;
+}
+
+/* valid[s] += number of variants in [v0, v0 + chunk) with a call for sample
+ * s; grid (ceil(n_samples / 256), ceil(n_var / chunk)).  missing = n_var -
+ * valid is taken on the host. */
+static __global__ void __launch_bounds__(256) k_qa_samples(const uint64_t *__restrict__ nat,
+                                                           int64_t n_var, int32_t w64,
+                                                           int32_t n_samples, int32_t chunk,
+                                                           int32_t *__restrict__ valid_count) {
+    const int32_t s = blockIdx.x * 256 + threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.y * chunk;
+    const int64_t v1 = v0 + chunk < n_var ? v0 + chunk : n_var;
+    if (s >= n_samples) return;
+    const int32_t w = s >> 6;
+    const uint64_t bit = 1ull << (s & 63);
+    int32_t c = 0;
+    for (int64_t v = v0; v < v1; v++) c += (nat[(v * 3 + 2) * w64 + w] & bit) ? 1 : 0;
+    if (c) atomicAdd(&valid_count[s], c);
+}
+
+/* dst variant i = src variant idx[i]: its three plane rows of w64 words. */
+static __global__ void __launch_bounds__(256) k_gather_variants(const uint64_t *__restrict__ src,
+                                                                const int64_t *__restrict__ idx,
+                                                                int64_t n_keep, int32_t w64,
+                                                                uint64_t *__restrict__ dst) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t per = (int64_t)3 * w64;
+    if (t >= n_keep * per) return;
+    const int64_t i = t / per, o = t - i * per;
+    dst[i * per + o] = src[idx[i] * per + o];
 }
 
 } // namespace gepi

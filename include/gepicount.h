@@ -132,6 +132,25 @@ int gepi_scan(gepi_panel *panel, const uint64_t *mask_ctrl, const uint64_t *mask
               gepi_scan_stats *stats);
 /* Reads the panel's planes back in reader layout (testing aid). */
 int gepi_panel_download(gepi_panel *panel, uint64_t *bit_data, uint64_t *valid_data);
+/* Missing-call QA on the device: the counts of checkQuality
+ * (src/reader/PlinkReader.h:237-335).  variant_missing[n_variants] and
+ * sample_missing[n_samples] (either may be NULL) receive missing calls; a
+ * variant counts as bad when (float)missing / n_samples > max_miss_snps, a
+ * sample when its fraction of missing allele rows > max_miss_samples, the
+ * reference's tests (PlinkReader.h:285, :316).                               */
+int gepi_panel_qa(gepi_panel *panel, float max_miss_snps, float max_miss_samples,
+                  int32_t *variant_missing, int32_t *sample_missing, int64_t *n_bad_variants,
+                  int64_t *n_bad_samples);
+/* checkQuality(remove = true), which the reference declares but does not
+ * implement (PlinkReader.h:236): *out is a new resident panel holding only
+ * the variants within max_miss_snps, in order; kept_variants[n_variants]
+ * (may be NULL) receives their original indices and *n_kept their number;
+ * keep_samples[ceil(n_samples/64)] (may be NULL) the mask of samples within
+ * max_miss_samples, to be ANDed into the class masks of later scans.  Both
+ * fractions are taken on the unfiltered panel.                               */
+int gepi_panel_filter(gepi_panel *panel, float max_miss_snps, float max_miss_samples,
+                      gepi_panel **out, int64_t *kept_variants, int64_t *n_kept,
+                      uint64_t *keep_samples);
 
 /* ---- batched scans ---------------------------------------------------------
  * Many scans (panels of any size, phenotype vectors) advance together in
