@@ -146,7 +146,7 @@ def run_mix(args):
     ops = var_pairs_samples * 9 * 2.0
     lat = sorted(lat)
     line = {
-        "metric": "C4 mix: 64 uneven scans batched, evals/s", "value": evals / statistics.mean(bat_t),
+        "metric": "SNP-pair x Boolean-fn evals/s, 64 uneven scans batched", "value": evals / statistics.mean(bat_t),
         "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1000.0 * total / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp4 (e2m1, fp32 accumulate)", "data": "synthetic",
@@ -248,7 +248,7 @@ def run_serve(args):
         identical &= bool(np.array_equal(alone, results[i][0]))
     srv.close()
     line = {
-        "metric": "C5 serving: evals/s under Poisson arrivals; p50/p99 request latency",
+        "metric": "SNP-pair x Boolean-fn evals/s served under Poisson arrivals; p50/p99 request latency",
         "value": evals / wall, "unit": "evals/s", "n_gpus": 1, "steps": len(reqs), "warmup": 4,
         "ms_per_step": 1000.0 * wall / len(reqs), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp4 (e2m1, fp32 accumulate)", "data": "synthetic",

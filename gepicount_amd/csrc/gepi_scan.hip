@@ -678,8 +678,9 @@ int panel_missing(gepi_panel *p, std::vector<int32_t> &var_miss, std::vector<int
     return GEPI_OK;
 }
 
-/* The reference's tests (PlinkReader.h:285, :316) on row counts: a row
- * fraction equals the variant fraction since both alleles share calls. */
+/* The reference's tests (src/reader/PlinkReader.h:318-319 per variant,
+ * :281-282 per sample), which count allele rows: both rows of a variant share
+ * its calls, so a sample misses 2 rows per missing variant out of 2*n_var. */
 inline bool variant_bad(int32_t miss, int32_t n_samples, float thr) {
     return n_samples > 0 && (float)miss / (float)n_samples > thr;
 }

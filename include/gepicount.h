@@ -137,7 +137,7 @@ int gepi_panel_download(gepi_panel *panel, uint64_t *bit_data, uint64_t *valid_d
  * sample_missing[n_samples] (either may be NULL) receive missing calls; a
  * variant counts as bad when (float)missing / n_samples > max_miss_snps, a
  * sample when its fraction of missing allele rows > max_miss_samples, the
- * reference's tests (PlinkReader.h:285, :316).                               */
+ * reference's tests (PlinkReader.h:318-319, :281-282).                               */
 int gepi_panel_qa(gepi_panel *panel, float max_miss_snps, float max_miss_samples,
                   int32_t *variant_missing, int32_t *sample_missing, int64_t *n_bad_variants,
                   int64_t *n_bad_samples);
