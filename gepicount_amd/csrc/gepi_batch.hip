@@ -155,13 +155,15 @@ int Engine::grow_desc(size_t n) {
 
 size_t Engine::estimate_footprint(const gepi_panel *p, int64_t topmost) {
     const int64_t n_var_pad = ((p->n_var + TM - 1) / TM) * TM;
-    const int64_t n_k4 = (p->n_samples + 127) / 128 + 2;
+    /* super-steps of a scan whose classes hold every sample (a sample in
+     * both masks, counted twice, is not assumed) */
+    const int64_t n_u = ((p->n_samples + 127) / 128 + 2) / 3 + 2;
     const int64_t R = 2 * p->n_var;
     const int64_t k = std::min<int64_t>(topmost, 5 * (R * (R + 1) / 2));
     const unsigned long long cap =
         std::max<unsigned long long>((unsigned long long)k + TILE_EVALS_MAX + 1024, 4ull * k);
     const unsigned long long pcap = std::max<unsigned long long>(cap, 1ull << 20);
-    return round_block((size_t)n_k4 * n_var_pad * 3 * 16) + round_block(cap * sizeof(Cand)) +
+    return round_block((size_t)n_u * n_var_pad * 3 * 64) + round_block(cap * sizeof(Cand)) +
            round_block(pcap * sizeof(Pend)) + round_block((size_t)4 * p->w64 * sizeof(CompressWord)) +
            round_block(sizeof(ScanState));
 }
@@ -250,7 +252,8 @@ int Engine::add(ScanItem *it) {
     if (getenv("GEPI_PEND_CAP"))
         it->pcap = std::max<unsigned long long>(env_ull("GEPI_PEND_CAP", 0),
                                                 (unsigned long long)TILE_EVALS_MAX + 1024);
-    const size_t lay = (size_t)std::max(it->n_k4, 1) * it->n_var_pad * 3 * sizeof(uint4);
+    const int32_t n_u = f4_super(w0 / 4) + f4_super(w1 / 4); /* 64-byte rows each */
+    const size_t lay = (size_t)std::max(n_u, 1) * it->n_var_pad * 3 * 4 * sizeof(uint4);
     it->d_layout = (uint4 *)dalloc(lay);
     it->d_cw = (CompressWord *)dalloc(it->h_cw.size() * sizeof(CompressWord));
     it->d_pend = (Pend *)dalloc(it->pcap * sizeof(Pend));
@@ -274,7 +277,8 @@ int Engine::add(ScanItem *it) {
     if (it->n_k4 > 0) {
         const int64_t rows = it->n_var_pad * 3;
         k_class_f4<<<(unsigned)((rows + 255) / 256), 256, 0, stream>>>(
-            p->d_nat, (int32_t)it->n_var, p->w64, it->d_cw, rows, it->ks0, it->d_layout);
+            p->d_nat, (int32_t)it->n_var, p->w64, it->d_cw, rows, f4_super(it->ks0),
+            it->d_layout);
         HIP_TRY(hipGetLastError());
     }
     it->max_band = std::max<int64_t>(64, (int64_t)(2.0e9 / (double)F4_TILE_EVALS));

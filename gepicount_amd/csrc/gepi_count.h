@@ -16,14 +16,15 @@
  *               16383): both classes in one pass over K.  64x64-variant
  *               workgroup tiles, 4 waves of 32x32, 64-sample K steps double
  *               buffered in LDS.
- * k_count_f4    matrix-core path on fp4 (e2m1) operands: packed bits staged
- *               in LDS, each wave expands its own fragments in registers (one
- *               bit -> one 0.5-valued nibble, 7 VALU ops per 32 samples) for
- *               v_mfma_scale_f32_16x16x128_f8f6f4 with block scale 2, so a
- *               product is exactly 1.0 and the fp32 accumulators hold exact
- *               counts (< 2^24).  Classes use separate accumulators (no class
- *               size limit).  64x32-variant workgroup tiles, 4 waves of 32x16,
- *               128-sample K steps double buffered in LDS.
+ * k_count_f4    matrix-core path on fp4 (e2m1) operands: the layout keeps 24
+ *               samples per 32-bit word, at nibble bits 0-2, so that one AND
+ *               per dword makes a fragment (a set bit at nibble bit p is
+ *               0.5 * 2^p); v_mfma_scale_f32_16x16x128_f8f6f4 with block
+ *               scale 2^(1-p) makes every product exactly 1.0, and the fp32
+ *               accumulators hold exact counts (< 2^24).  Classes use
+ *               separate accumulators (no class size limit).  64x32-variant
+ *               workgroup tiles, 4 waves of 32x16, 384-sample super-steps
+ *               (three K steps from the same words) double buffered in LDS.
  */
 #ifndef GEPI_COUNT_H
 #define GEPI_COUNT_H
@@ -299,10 +300,16 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int F4_THREADS = 256; /* 4 waves, 2 x 2 of 32 x 16 variants             */
 constexpr int TB4 = 32;         /* B variants per workgroup tile (A: TM = 64)     */
-constexpr int F4_QA = TM * 3;   /* uint4 (128-sample rows) of A per K step        */
-constexpr int F4_Q = F4_QA + TB4 * 3; /* uint4 per K step: A rows then B rows    */
+constexpr int F4_RA = TM * 3;   /* A plane rows of a tile                         */
+constexpr int F4_ROWS = F4_RA + TB4 * 3; /* plane rows of a tile: A, then B      */
+constexpr int F4_PITCH = 5;     /* uint4 per LDS row: 4 of data, 1 of padding that
+                                   spreads the 16-byte fragment reads of 8
+                                   consecutive variants over distinct banks     */
 constexpr int F4_FMT = 4;       /* operand format code of e2m1                    */
-constexpr int F4_SCALE = 128;   /* E8M0 block scale 2^1: 0.5 * 2 = 1.0           */
+
+/* fp4 layout geometry: a super-step is 3 K steps of 128 samples; each class
+ * starts on a super-step, class 0 at 0 and class 1 at f4_super(ks0).       */
+__host__ __device__ inline int32_t f4_super(int32_t ks) { return (ks + 2) / 3; }
 
 /* Workgroup tile t = ta*(ta+1) + tb, 0 <= tb <= 2*ta+1: A variants
  * [64ta, 64ta+64) against B variants [32tb, 32tb+32). */
@@ -315,20 +322,21 @@ __host__ __device__ inline void tile_of_f4(int64_t t, int32_t &ta, int32_t &tb) 
     tb = (int32_t)(t - a * (a + 1));
 }
 
-/* 32 packed samples -> the 4 dwords of one lane's fp4 fragment: nibble n of
- * dword d is bit 4n+d of w (0x1 = e2m1 0.5, 0x0 = 0).  The order of samples
- * within a K step does not change a dot product, and A and B fragments use
- * the same order.                                                          */
-__device__ __forceinline__ v8i expand_f4(uint32_t w) {
-    return v8i{(int32_t)(w & 0x11111111u), (int32_t)((w >> 1) & 0x11111111u),
-               (int32_t)((w >> 2) & 0x11111111u), (int32_t)((w >> 3) & 0x11111111u), 0, 0, 0, 0};
+/* One lane's fp4 fragment of K step p of a super-step: the lane's four
+ * layout dwords masked to nibble bit p.  A set bit is e2m1 0.5 * 2^p; with
+ * block scale 2^(1-p) on both operands every product of two set bits is
+ * exactly 1.0.  The order of samples within a K step does not change a dot
+ * product, and A and B fragments use the same order.                      */
+__device__ __forceinline__ v8i frag_f4(const uint4 &w, uint32_t m) {
+    return v8i{(int32_t)(w.x & m), (int32_t)(w.y & m), (int32_t)(w.z & m), (int32_t)(w.w & m),
+               0, 0, 0, 0};
 }
 
 /* One 64 x 32-variant workgroup tile of the fp4 path (tile numbering as in
  * tile_of_f4).  Called by every thread of a 256-thread workgroup.          */
 template <int FFUN>
 __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) {
-    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_Q];
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_ROWS * F4_PITCH];
     __shared__ PendStage stage;
     stage_init(stage);
     int32_t ta, tb;
@@ -339,67 +347,85 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
     const int wa = wave >> 1, wb = wave & 1;
     const int l16 = lane & 15, lg = lane >> 4;
 
-    /* Staging.  A K step holds 288 rows of 128 packed samples (one uint4
-     * each): A rows 0..191 (variant 64ta + r/3, plane r%3), then B rows.
-     * Thread t copies row t, and threads t < 32 also row 256 + t.           */
-    const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
-    const int64_t sstride = (int64_t)g.n_var_pad * 3; /* uint4 per K step */
-    const uint4 *src0 = tid < F4_QA ? base + (int64_t)ta * F4_QA + tid
-                                    : base + (int64_t)tb * (TB4 * 3) + (tid - F4_QA);
-    const bool has1 = tid < F4_Q - F4_THREADS;
-    const uint4 *src1 = base + (int64_t)tb * (TB4 * 3) + (F4_THREADS - F4_QA) + (has1 ? tid : 0);
-    const int last = g.n_ks > 0 ? g.n_ks - 1 : 0; /* n_ks == 0: no K loop, step 0 exists */
+    /* super-steps: class 0 [0, u1), class 1 [u1, n_u) */
+    const int32_t ks1 = g.n_ks - g.ks0;
+    const int32_t u1 = f4_super(g.ks0), n_u = u1 + f4_super(ks1);
+    const int32_t last = n_u > 0 ? n_u - 1 : 0; /* n_u == 0: no K loop, super-step 0 exists */
 
-    lds[0][tid] = src0[0];
-    if (has1) lds[0][F4_THREADS + tid] = src1[0];
-    uint4 pf0, pf1 = make_uint4(0u, 0u, 0u, 0u);
-    {
-        const int64_t o1 = (int64_t)(last < 1 ? last : 1) * sstride;
-        pf0 = src0[o1];
-        if (has1) pf1 = src1[o1];
-    }
+    /* Staging.  A super-step of a tile is 288 plane rows of 64 bytes (4
+     * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
+     * contiguous in the layout.  uint4 i < 768 of the A rows and i < 384 of
+     * the B rows; thread t takes A i = t + 256j (j < 3) and B i = t, t + 256
+     * (t < 128), i.e. row i/4, column t%4.                                  */
+    const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
+    const int64_t ustride = (int64_t)g.n_var_pad * 3 * 4; /* uint4 per super-step */
+    const uint4 *gA = base + (int64_t)ta * (F4_RA * 4) + tid;
+    const uint4 *gB = base + (int64_t)tb * (TB4 * 3 * 4) + tid;
+    const bool hasB1 = tid < 128;
+    const int dA = (tid >> 2) * F4_PITCH + (tid & 3);
+    const int dB = (F4_RA + (tid >> 2)) * F4_PITCH + (tid & 3);
+    constexpr int D64 = 64 * F4_PITCH; /* 64 rows further down */
+    uint4 pa0, pa1, pa2, pb0, pb1 = make_uint4(0u, 0u, 0u, 0u);
+    auto fetch = [&](int32_t u) {
+        const int64_t o = (int64_t)u * ustride;
+        pa0 = gA[o];
+        pa1 = gA[o + 256];
+        pa2 = gA[o + 512];
+        pb0 = gB[o];
+        if (hasB1) pb1 = gB[o + 256];
+    };
+    auto put = [&](uint4 *W) {
+        W[dA] = pa0;
+        W[dA + D64] = pa1;
+        W[dA + 2 * D64] = pa2;
+        W[dB] = pb0;
+        if (hasB1) W[dB + D64] = pb1;
+    };
+    fetch(0);
+    put(lds[0]);
+    fetch(last < 1 ? last : 1);
     __syncthreads();
 
-    /* lane's fragment words (dwords within a step buffer): A (i, p) is row
-     * (32wa + 16i + l16)*3 + p, B q is row 192 + (16wb + l16)*3 + q; the lane
-     * takes 32-sample word lg of the row.                                   */
-    const int offA = (32 * wa + l16) * 3 * 4 + lg;
-    const int offB = (F4_QA + (16 * wb + l16) * 3) * 4 + lg;
-    /* K steps [s_begin, s_end) into one class's accumulators; the staging
-     * pipeline runs on across the two calls.                                */
-    auto steps = [&](v4f (&acc)[2][3][3], int32_t s_begin, int32_t s_end) {
-        for (int32_t s = s_begin; s < s_end; s++) {
-            const uint32_t *L = reinterpret_cast<const uint32_t *>(lds[s & 1]);
-            uint32_t wA[2][3], wB[3];
+    /* lane's fragment rows: A (i, p) is row (32wa + 16i + l16)*3 + p, B q is
+     * row 192 + (16wb + l16)*3 + q; the lane takes 16-byte column lg.      */
+    const int fA = (32 * wa + l16) * 3 * F4_PITCH + lg;
+    const int fB = (F4_RA + (16 * wb + l16) * 3) * F4_PITCH + lg;
+    /* super-step u into one class's accumulators, its first np K steps
+     * (the rest of a class's last super-step is padding); the staging
+     * pipeline runs on across the two calls.                               */
+    auto step = [&](v4f (&acc)[2][3][3], int32_t u, int32_t np) {
+        const uint4 *L = lds[u & 1];
+        uint4 wA[2][3], wB[3];
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+        for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int p = 0; p < 3; p++) wA[i][p] = L[offA + (48 * i + p) * 4];
+            for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
 #pragma unroll
-            for (int q = 0; q < 3; q++) wB[q] = L[offB + q * 4];
-            /* stage step s+1 (a harmless extra copy on the last step: that
-             * buffer is not read again), prefetch step s+2 */
-            uint4 *W = lds[(s & 1) ^ 1];
-            W[tid] = pf0;
-            if (has1) W[F4_THREADS + tid] = pf1;
-            const int64_t on = (int64_t)(s + 2 <= last ? s + 2 : last) * sstride;
-            pf0 = src0[on];
-            if (has1) pf1 = src1[on];
-            v8i b[3];
+        for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
+        /* stage super-step u+1 (a harmless extra copy on the last one: that
+         * buffer is not read again), prefetch u+2 */
+        put(lds[(u & 1) ^ 1]);
+        fetch(u + 2 <= last ? u + 2 : last);
 #pragma unroll
-            for (int q = 0; q < 3; q++) b[q] = expand_f4(wB[q]);
+        for (int k = 0; k < 3; k++) {
+            if (k < np) {
+                const uint32_t m = 0x11111111u << k;
+                v8i b[3];
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+                for (int q = 0; q < 3; q++) b[q] = frag_f4(wB[q], m);
 #pragma unroll
-                for (int p = 0; p < 3; p++) {
-                    const v8i a = expand_f4(wA[i][p]);
+                for (int i = 0; i < 2; i++)
 #pragma unroll
-                    for (int q = 0; q < 3; q++)
-                        acc[i][p][q] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                            a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, F4_SCALE, 0, F4_SCALE);
-                }
-            __syncthreads();
+                    for (int p = 0; p < 3; p++) {
+                        const v8i a = frag_f4(wA[i][p], m);
+#pragma unroll
+                        for (int q = 0; q < 3; q++)
+                            acc[i][p][q] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                                a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, 128 - k, 0, 128 - k);
+                    }
+            }
         }
+        __syncthreads();
     };
     v4f acc0[2][3][3], acc1[2][3][3];
 #pragma unroll
@@ -408,14 +434,20 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
         for (int p = 0; p < 3; p++)
 #pragma unroll
             for (int q = 0; q < 3; q++) acc0[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-    steps(acc0, 0, g.ks0);
+    for (int32_t u = 0; u < u1; u++) {
+        const int32_t r = g.ks0 - 3 * u;
+        step(acc0, u, r < 3 ? r : 3);
+    }
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int p = 0; p < 3; p++)
 #pragma unroll
             for (int q = 0; q < 3; q++) acc1[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
-    steps(acc1, g.ks0, g.n_ks);
+    for (int32_t u = u1; u < n_u; u++) {
+        const int32_t r = ks1 - 3 * (u - u1);
+        step(acc1, u, r < 3 ? r : 3);
+    }
 
     /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
      * variant 32wa + 16i + 4lg + r against B variant 16wb + l16.           */

@@ -51,7 +51,8 @@ struct ScanState {
 struct CountArgs {
     const uint32_t *planes;   /* popcount layout [n_var_pad][3][wc] dwords          */
     const uint64_t *planes_k; /* int8: [n_ks][n_var_pad][3] 64-sample words;       
-                               * fp4: [n_ks][n_var_pad*3] 128-sample rows          */
+                               * fp4: [n_u][n_var_pad*3][4] uint4 super-step rows
+                               * (3 K steps of 128 samples; see k_class_f4)        */
     int64_t tile_begin;       /* first linear tile of the band                      */
     int32_t n_tiles;          /* tiles in the band (= grid size)                    */
     int32_t n_var;
@@ -60,7 +61,8 @@ struct CountArgs {
     int32_t w0;               /* class-0 dwords (multiple of 4)                     */
     int32_t wused;            /* dwords holding data (multiple of 4)                */
     int32_t n_ks;             /* K steps: 64 samples (int8), 128 samples (fp4)      */
-    int32_t ks0;              /* K steps of class 0                                 */
+    int32_t ks0;              /* K steps of class 0 (fp4: class 1 starts at
+                               * super-step f4_super(ks0))                         */
     int32_t cls0_high;        /* MFMA: class 0 encoded as 0x80 (else class 1 is)    */
     int32_t pad;
     ScanState *st;

@@ -275,9 +275,12 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     if (path != 3 &&
         (rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK)
         return rc;
+    /* int8: n_ks 64-sample words per plane row; fp4: n_u 64-byte super-step rows */
+    const int32_t n_u = f4_super(w0 / 4) + f4_super(w1 / 4);
     if (path >= 2 &&
         (rc = grow(&p->d_clsk, &p->clsk_bytes,
-                   (size_t)std::max(n_ks, 1) * n_var_pad * 3 * (path == 3 ? 16 : 8))) != GEPI_OK)
+                   path == 3 ? (size_t)std::max(n_u, 1) * n_var_pad * 3 * 64
+                             : (size_t)std::max(n_ks, 1) * n_var_pad * 3 * 8)) != GEPI_OK)
         return rc;
     if ((rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK) return rc;
     /* fp4 path: per 32-sample word of the panel, the compression of each
@@ -335,7 +338,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         const int64_t rows = n_var_pad * 3;
         if (n_ks > 0) {
             k_class_f4<<<(unsigned)((rows + 255) / 256), 256, 0, p->stream>>>(
-                p->d_nat, (int32_t)n_var, p->w64, p->d_cw, rows, w0 / 4,
+                p->d_nat, (int32_t)n_var, p->w64, p->d_cw, rows, f4_super(w0 / 4),
                 reinterpret_cast<uint4 *>(p->d_clsk));
             HIP_TRY(hipGetLastError());
         }

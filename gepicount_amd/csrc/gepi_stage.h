@@ -11,16 +11,16 @@
  *                  popcount layout [n_var_pad][3][wc] dwords.
  * k_transpose_k    popcount layout -> MFMA layout [n_ks][n_var_pad][3] of
  *                  64-sample words, so that one K step of a tile is contiguous.
- * k_transpose_f4   popcount layout -> fp4 MFMA layout [n_k4][n_var_pad*3] of
- *                  128-sample rows (uint4), one K step of a tile contiguous.
  * k_qa_variants    missing calls per variant (one wave per variant)
  * k_qa_samples     missing calls per sample (a workgroup per 256 samples and
  *                  a chunk of variants; partial counts added atomically)
  * k_gather_variants  copies the planes of selected variants to a new panel
  * k_class_f4       panel -> fp4 MFMA layout directly: per plane row, the
  *                  samples of each class are compressed (stable) out of the
- *                  file order with mask-driven bit compression, and streamed
- *                  out in 128-sample rows (class 0 steps, then class 1).
+ *                  file order with mask-driven bit compression, cut into
+ *                  128-sample K steps and written three K steps at a time as
+ *                  64-byte super-step rows (class 0, then class 1 from the
+ *                  next super-step).
  * The QA kernels replace the loops of checkQuality (src/reader/
  * PlinkReader.h:237-335), which count missing calls on the host.
  */
@@ -109,27 +109,6 @@ static __global__ void __launch_bounds__(256) k_transpose_k(const uint32_t *__re
     }
 }
 
-/* rows = n_var_pad*3 (a multiple of 64); grid (rows / 64, ceil(n_k4 / 16)). */
-static __global__ void __launch_bounds__(256) k_transpose_f4(const uint32_t *__restrict__ cls,
-                                                       int32_t rows, int32_t wc, int32_t n_k4,
-                                                       uint4 *__restrict__ out) {
-    __shared__ uint4 tile[64][17];
-    const int32_t r0 = blockIdx.x * 64, s0 = blockIdx.y * 16;
-    for (int32_t idx = threadIdx.x; idx < 64 * 16; idx += blockDim.x) {
-        const int32_t r = idx >> 4, k = idx & 15;
-        const int32_t s = s0 + k;
-        uint4 val = make_uint4(0u, 0u, 0u, 0u);
-        if (s < n_k4) val = *reinterpret_cast<const uint4 *>(cls + (int64_t)(r0 + r) * wc + 4 * s);
-        tile[r][k] = val;
-    }
-    __syncthreads();
-    for (int32_t idx = threadIdx.x; idx < 64 * 16; idx += blockDim.x) {
-        const int32_t k = idx >> 6, r = idx & 63;
-        const int32_t s = s0 + k;
-        if (s < n_k4) out[(int64_t)s * rows + r0 + r] = tile[r][k];
-    }
-}
-
 /* Compression of one 32-bit sample word for one class: x & m moved to the
  * low popc(m) bits, order kept, by five conditional shifts whose move masks
  * depend on m only (Hacker's Delight, "compress").  Built on the host once
@@ -172,20 +151,64 @@ __device__ __forceinline__ uint32_t compress_word(uint32_t x, const CompressWord
     return x;
 }
 
-/* A stream of bits appended in order and written as 128-bit rows
- * out[step * rows + r].  Every thread of a launch appends the same counts in
- * the same order, so the bookkeeping is uniform.                            */
-struct BitStream {
-    uint64_t acc;
-    int32_t na, nq;
-    int64_t step;
-    uint32_t q0, q1, q2, q3;
+/* Bit n of the low byte of x moved to bit 4n (three shift-or-mask rounds). */
+__device__ __forceinline__ uint32_t spread8(uint32_t x) {
+    x &= 0xFFu;
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x;
+}
 
-    __device__ __forceinline__ void init(int64_t s0) {
+/* One 64-byte super-step row of the fp4 layout from the row's three
+ * 128-sample K steps k0, k1, k2 (32-bit word g of a K step holds its samples
+ * 32g .. 32g+31): dword d of 16-byte group g holds, at bit 4n + p, sample
+ * 32g + 8d + n of K step p.  Bit 3 of every nibble stays zero.            */
+__device__ __forceinline__ uint4 super_group(uint32_t a, uint32_t b, uint32_t c) {
+    uint4 o;
+    o.x = spread8(a) | (spread8(b) << 1) | (spread8(c) << 2);
+    o.y = spread8(a >> 8) | (spread8(b >> 8) << 1) | (spread8(c >> 8) << 2);
+    o.z = spread8(a >> 16) | (spread8(b >> 16) << 1) | (spread8(c >> 16) << 2);
+    o.w = spread8(a >> 24) | (spread8(b >> 24) << 1) | (spread8(c >> 24) << 2);
+    return o;
+}
+
+/* A stream of bits appended in order, cut into 128-sample K steps, and
+ * written three K steps at a time as super-step rows
+ * out[(u * rows + r) * 4 + g].  Every thread of a launch appends the same
+ * counts in the same order, so the bookkeeping is uniform.  The completed
+ * K steps of the current super-step advance through k0, k1 (an indexed
+ * array would be placed in scratch memory).                                */
+struct SuperStream {
+    uint64_t acc;
+    int32_t na, nq, np;
+    int64_t u;
+    uint32_t q0, q1, q2, q3;
+    uint4 k0, k1;
+
+    __device__ __forceinline__ void init(int64_t u0) {
         acc = 0;
-        na = nq = 0;
-        step = s0;
+        na = nq = np = 0;
+        u = u0;
         q0 = q1 = q2 = q3 = 0;
+        k0 = k1 = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __device__ __forceinline__ void kstep(const uint4 &kr, uint4 *out, int64_t rows, int64_t r) {
+        if (np == 0) {
+            k0 = kr;
+        } else if (np == 1) {
+            k1 = kr;
+        } else {
+            if (r < rows) {
+                uint4 *dst = out + (u * rows + r) * 4;
+                dst[0] = super_group(k0.x, k1.x, kr.x);
+                dst[1] = super_group(k0.y, k1.y, kr.y);
+                dst[2] = super_group(k0.z, k1.z, kr.z);
+                dst[3] = super_group(k0.w, k1.w, kr.w);
+            }
+            u++;
+        }
+        np = np == 2 ? 0 : np + 1;
     }
     __device__ __forceinline__ void emit(uint32_t w, uint4 *out, int64_t rows, int64_t r) {
         q0 = q1;
@@ -193,8 +216,7 @@ struct BitStream {
         q2 = q3;
         q3 = w;
         if (++nq == 4) {
-            if (r < rows) out[step * rows + r] = make_uint4(q0, q1, q2, q3);
-            step++;
+            kstep(make_uint4(q0, q1, q2, q3), out, rows, r);
             nq = 0;
         }
     }
@@ -208,7 +230,7 @@ struct BitStream {
             na -= 32;
         }
     }
-    /* zero-pads the last 128-bit row of the stream */
+    /* zero-pads the last K step and the last super-step of the stream */
     __device__ __forceinline__ void finish(uint4 *out, int64_t rows, int64_t r) {
         if (na > 0) {
             emit((uint32_t)acc, out, rows, r);
@@ -216,26 +238,28 @@ struct BitStream {
             na = 0;
         }
         while (nq != 0) emit(0u, out, rows, r);
+        while (np != 0) kstep(make_uint4(0u, 0u, 0u, 0u), out, rows, r);
     }
 };
 
 /* One thread per plane row r = v*3 + p of [0, rows) (rows = n_var_pad * 3;
  * rows of padding variants v >= n_var are zero).  cw: [2][2*w64]
- * CompressWord, class 0 then class 1.  Class 0 fills K steps [0, ks0),
- * class 1 steps [ks0, ks0 + ceil(n1 / 128)).  The panel rows of a
- * workgroup are staged through LDS 8 words at a time (coalesced reads).     */
+ * CompressWord, class 0 then class 1.  Class 0 fills super-steps [0, u1)
+ * (u1 = f4_super(its K steps)), class 1 the super-steps from u1 on.  The
+ * panel rows of a workgroup are staged through LDS 8 words at a time
+ * (coalesced reads).                                                       */
 constexpr int CLS_CHUNK = 8;
 static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
                                                    int32_t w64, const CompressWord *__restrict__ cw,
-                                                   int64_t rows, int32_t ks0,
+                                                   int64_t rows, int32_t u1,
                                                    uint4 *__restrict__ out) {
     __shared__ uint64_t srow[256][CLS_CHUNK + 1];
     const int64_t r0 = (int64_t)blockIdx.x * 256;
     const int64_t r = r0 + threadIdx.x;
     const int64_t real_rows = (int64_t)n_var * 3;
-    BitStream s0, s1;
+    SuperStream s0, s1;
     s0.init(0);
-    s1.init(ks0);
+    s1.init(u1);
     const CompressWord *c0 = cw, *c1 = cw + 2 * w64;
     for (int32_t base = 0; base < w64; base += CLS_CHUNK) {
         __syncthreads();
