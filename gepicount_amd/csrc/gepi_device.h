@@ -241,17 +241,54 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, con
  * so log10 P(obs) - 1e-6 <= thr needs
  *     T = (-(thr + 1e-6) ln 10 - ln(4yz) / 2) / m   (y, z clamped to >= 1);
  * its survivors are then checked exactly against the log-factorial table.
- * The test is float(D)^2 > K*y*z*(1 - 2^-16) * q: the float error of both
- * sides is below 2^-21 relative, so every table meeting the exact bound
- * passes, and D == 0 (an independent table, I == 0) fails whenever K > 0;
- * K == 0 admits everything.  D and xx are linear in the three per-row-pair
- * quantities (N11, NA, NB): D_f = {Dn, Db-Dn, Da-Dn, Dn-Da-Db, Da+Db-2Dn}
- * with Dq = q0*z - q1*y, and likewise xx_f from S = q0 + q1, computed with
- * 24-bit products (all counts < 2^16).
+ * The test is s = fd^2 - RN(K*y*z*(1 - 2^-16) * q) > bias with fd = float(D)
+ * squared exactly inside one fma: the float error of the right side is
+ * below 2^-21 relative and the left side is exact, so every table meeting
+ * the exact bound passes; D == 0 (an independent table, I == 0) fails
+ * whenever K > 0, and K == 0 (bias -1) admits everything.  D and xx are
+ * linear in the three per-row-pair quantities (N11, NA, NB):
+ * D_f = {Dn, Db-Dn, Da-Dn, Dn-Da-Db, Da+Db-2Dn} with Dq = q0*z - q1*y
+ * (24-bit products, all counts < 2^16), and likewise xx_f from S = q0 + q1.
+ * A row pair is flagged when the largest of its five s exceeds the bias;
+ * the bits of the functions are rebuilt, from the same expressions, only
+ * for the lanes with a flagged row pair.
  *
  * ROLLED: a runtime loop over the four row pairs keeps the code of callers
  * with many pairs per thread small; its operands advance through rotating
  * registers (an indexed array would be placed in scratch memory).          */
+struct RowTest {
+    float kyz, mf, bias;
+    /* s of the five functions of one row pair */
+    __device__ __forceinline__ void s5(int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db,
+                                       float Sb, float (&s)[5]) const {
+        const int32_t Dab = Da + Db;
+        const float Sab = Sa + Sb;
+        const int32_t D[5] = {Dn, Db - Dn, Da - Dn, Dn - Dab, Dab - 2 * Dn};
+        const float X[5] = {Sn, Sb - Sn, Sa - Sn, (mf - Sab) + Sn, Sab - 2.0f * Sn};
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            const float r = kyz * (X[f] * (mf - X[f]));
+            const float fd = (float)D[f];
+            s[f] = __builtin_fmaf(fd, fd, -r);
+        }
+    }
+    __device__ __forceinline__ bool any(int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db,
+                                        float Sb) const {
+        float s[5];
+        s5(Dn, Sn, Da, Sa, Db, Sb, s);
+        return fmaxf(fmaxf(fmaxf(s[0], s[1]), fmaxf(s[2], s[3])), s[4]) > bias;
+    }
+    __device__ __forceinline__ uint32_t bits(int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db,
+                                             float Sb) const {
+        float s[5];
+        s5(Dn, Sn, Da, Sa, Db, Sb, s);
+        uint32_t b = 0;
+#pragma unroll
+        for (int f = 0; f < 5; f++) b |= s[f] > bias ? (1u << f) : 0u;
+        return b;
+    }
+};
+
 template <int FFUN, bool ROLLED>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                             int32_t A, int32_t B, const int32_t (&c)[2][3][3]) {
@@ -267,31 +304,18 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
             if (T > 0.0f) K = expm1f(T) * (1.0f - 1e-5f);
         }
     }
-    const float kyz = K * (float)__mul24(y, z) * (1.0f - 1.0f / 65536.0f);
-    const float bias = K == 0.0f ? -1.0f : 0.0f;
+    RowTest rt;
+    rt.kyz = K * (float)__mul24(y, z) * (1.0f - 1.0f / 65536.0f);
+    rt.mf = mf;
+    rt.bias = K == 0.0f ? -1.0f : 0.0f;
     auto dq = [&](int32_t q0, int32_t q1) { return __mul24(q0, z) - __mul24(q1, y); };
     auto sq = [&](int32_t q0, int32_t q1) { return (float)(q0 + q1); };
-    /* 5-bit pass pattern of one row pair */
-    auto row_bits = [&](int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db, float Sb) {
-        const int32_t Dab = Da + Db;
-        const float Sab = Sa + Sb;
-        const int32_t D[5] = {Dn, Db - Dn, Da - Dn, Dn - Dab, Dab - 2 * Dn};
-        const float X[5] = {Sn, Sb - Sn, Sa - Sn, (mf - Sab) + Sn, Sab - 2.0f * Sn};
-        uint32_t bits = 0;
-#pragma unroll
-        for (int f = 0; f < 5; f++) {
-            const float q = X[f] * (mf - X[f]);
-            const float rhs = __builtin_fmaf(kyz, q, bias);
-            const float fd = (float)D[f];
-            bits |= (fd * fd > rhs) ? (1u << f) : 0u;
-        }
-        return bits;
-    };
     const int32_t Da0 = dq(c[0][0][2], c[1][0][2]), Da1 = dq(c[0][1][2], c[1][1][2]);
     const int32_t Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
     const float Sa0 = sq(c[0][0][2], c[1][0][2]), Sa1 = sq(c[0][1][2], c[1][1][2]);
     const float Sb0 = sq(c[0][2][0], c[1][2][0]), Sb1 = sq(c[0][2][1], c[1][2][1]);
-    uint32_t mask = 0;
+    /* row pair rp = 2ia + ib flagged in bit rp */
+    uint32_t rows = 0;
     if (ROLLED) {
         int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
         int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
@@ -299,7 +323,7 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
         float Sa = Sa0, Sb = Sb0, Sbo = Sb1;
 #pragma unroll 1
         for (int rp = 0; rp < 4; rp++) {
-            mask |= row_bits(dq(n0a, n1a), sq(n0a, n1a), Da, Sa, Db, Sb) << (5 * rp);
+            rows |= rt.any(dq(n0a, n1a), sq(n0a, n1a), Da, Sa, Db, Sb) ? 1u << rp : 0u;
             n0a = n0b; n0b = n0c; n0c = n0d;
             n1a = n1b; n1b = n1c; n1c = n1d;
             if (rp == 1) { Da = Da1; Sa = Sa1; }
@@ -307,13 +331,25 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
             float u = Sb; Sb = Sbo; Sbo = u;
         }
     } else {
-        mask = row_bits(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0) |
-               row_bits(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) << 5 |
-               row_bits(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) << 10 |
-               row_bits(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) << 15;
+        rows = (rt.any(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0) ? 1u : 0u) |
+               (rt.any(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) ? 2u : 0u) |
+               (rt.any(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) ? 4u : 0u) |
+               (rt.any(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) ? 8u : 0u);
     }
-    if (A == B) mask &= ~(0x1Fu << 5); /* (2A, 2A+1) is not a row pair */
-    if (mask) push_mask<FFUN>(g, ps, fc, A, B, c, mask);
+    if (A == B) rows &= ~2u; /* (2A, 2A+1) is not a row pair */
+    if (rows) {
+        /* rare: the function bits of the flagged row pairs */
+        uint32_t mask = 0;
+        if (rows & 1u)
+            mask |= rt.bits(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0);
+        if (rows & 2u)
+            mask |= rt.bits(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) << 5;
+        if (rows & 4u)
+            mask |= rt.bits(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) << 10;
+        if (rows & 8u)
+            mask |= rt.bits(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) << 15;
+        push_mask<FFUN>(g, ps, fc, A, B, c, mask);
+    }
 }
 
 } // namespace gepi
