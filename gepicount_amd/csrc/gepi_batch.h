@@ -3,8 +3,9 @@
  * and phenotype vectors of different sizes) advance together in rounds; a
  * round is three launches whatever the number of scans:
  *
- *   k_count_f4_batch  one band of fp4 count tiles per scan in a single grid:
- *                     a descriptor table maps workgroups to (scan, tile)
+ *   k_count_f4_batch  one band of fp4 count tiles per scan in one persistent
+ *                     grid: workgroups take (scan, tile) slots from per-XCD
+ *                     work counters through a descriptor table
  *   k_exact_batch     exact scores of each scan's filter survivors
  *                     (blockIdx.y = scan)
  *   k_topk_wg         one 1024-thread workgroup per scan: radix select of
@@ -27,36 +28,91 @@ struct BatchDesc {
     CountArgs g; /* tile_begin / n_tiles: the band                            */
 };
 
-/* Workgroup -> (scan, tile).  Workgroups are dispatched to the 8 XCDs round
+/* Slot -> (scan, tile).  Workgroups are dispatched to the 8 XCDs round
  * robin (XCD = blockIdx.x % 8).  Every scan's band is split into 8 contiguous
  * chunks, one per XCD (so each XCD's L2 sees neighbouring tiles), and each
- * XCD runs its chunks scan after scan: slot s = blockIdx.x / 8 of XCD x is
- * in scan i when xprefix[x][i] <= s < xprefix[x][i+1], at tile
- * tile_begin + xstart[x][i] + (s - xprefix[x][i]).  Chunk sizes follow the
- * round-robin dealing of the band, so every XCD gets the same number of
- * workgroups of every scan to within one.                                  */
+ * XCD runs its chunks scan after scan: slot s of XCD x (taken from the
+ * XCD's work counter) is in scan i when xprefix[x][i] <= s < xprefix[x][i+1],
+ * at tile tile_begin + xstart[x][i] + (s - xprefix[x][i]).  Chunk sizes
+ * follow the round-robin dealing of the band, so every XCD gets the same
+ * number of tiles of every scan to within one.                            */
 struct BatchMap {
     const int64_t *xprefix; /* [8][n + 1] */
     const int64_t *xstart;  /* [8][n]     */
+    uint32_t *wq;           /* 2 sets of 8 work counters                  */
     int32_t n;
+    int32_t wq_parity;      /* set of this launch                          */
 };
 
-template <int FFUN>
+struct F4BatchSrc {
+    const BatchDesc *d;
+    BatchMap map;
+    __device__ __forceinline__ F4Job grab(int32_t) const {
+        const int32_t x = blockIdx.x & 7;
+        const int64_t slot = atomicAdd(&map.wq[8 * map.wq_parity + x], 1u);
+        const int64_t *pre = map.xprefix + (int64_t)x * (map.n + 1);
+        F4Job j;
+        j.ok = slot < pre[map.n];
+        j.item = 0;
+        j.tile = 0;
+        if (!j.ok) return j;
+        int32_t lo = 0, hi = map.n - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= slot)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        j.item = lo;
+        j.tile = d[lo].g.tile_begin + map.xstart[(int64_t)x * map.n + lo] + (slot - pre[lo]);
+        return j;
+    }
+    __device__ __forceinline__ CountArgs args(int32_t item) const { return d[item].g; }
+};
+
+/* One (scan, tile) per workgroup: slot = blockIdx.x / 8 of XCD blockIdx.x % 8. */
+struct F4BatchOnceSrc {
+    const BatchDesc *d;
+    BatchMap map;
+    __device__ __forceinline__ F4Job grab(int32_t n) const {
+        const int32_t x = blockIdx.x & 7;
+        const int64_t slot = blockIdx.x >> 3;
+        const int64_t *pre = map.xprefix + (int64_t)x * (map.n + 1);
+        F4Job j;
+        j.ok = n == 0 && slot < pre[map.n];
+        j.item = 0;
+        j.tile = 0;
+        if (!j.ok) return j;
+        int32_t lo = 0, hi = map.n - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= slot)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        j.item = lo;
+        j.tile = d[lo].g.tile_begin + map.xstart[(int64_t)x * map.n + lo] + (slot - pre[lo]);
+        return j;
+    }
+    __device__ __forceinline__ CountArgs args(int32_t item) const { return d[item].g; }
+};
+
+/* PERSIST: workgroups take slots from the work counters (grid of a few
+ * workgroups per CU); else one slot per workgroup (grid = 8 x the largest
+ * XCD share). */
+template <int FFUN, bool PERSIST>
 __global__ void __launch_bounds__(F4_THREADS, 2)
     k_count_f4_batch(const BatchDesc *__restrict__ d, BatchMap map) {
-    const int32_t x = blockIdx.x & 7;
-    const int64_t slot = blockIdx.x >> 3;
-    const int64_t *pre = map.xprefix + (int64_t)x * (map.n + 1);
-    int32_t lo = 0, hi = map.n - 1;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= slot)
-            lo = mid;
-        else
-            hi = mid - 1;
+    if (PERSIST) {
+        zero_other_counters(map.wq, map.wq_parity);
+        const F4BatchSrc src{d, map};
+        count_f4_run<FFUN>(src);
+    } else {
+        const F4BatchOnceSrc src{d, map};
+        count_f4_run<FFUN>(src);
     }
-    const CountArgs g = d[lo].g;
-    count_f4_tile<FFUN>(g, g.tile_begin + map.xstart[(int64_t)x * map.n + lo] + (slot - pre[lo]));
 }
 
 struct ExactDesc {

@@ -55,6 +55,9 @@ int Engine::init(int dev) {
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&ev0_));
     HIP_TRY(hipEventCreate(&ev1_));
+    /* fp4 work counters: per objective launch, 2 sets of 8 */
+    HIP_TRY(hipMalloc(&d_wq_, sizeof(uint32_t) * 32));
+    HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
     return GEPI_OK;
 }
 
@@ -80,6 +83,8 @@ void Engine::release() {
     hipFree(d_edesc_);
     hipFree(d_tdesc_);
     hipFree(d_rep_);
+    hipFree(d_wq_);
+    d_wq_ = nullptr;
     d_edesc_ = nullptr;
     d_tdesc_ = nullptr;
     d_rep_ = nullptr;
@@ -406,6 +411,10 @@ int Engine::round(std::vector<ScanItem *> &finished) {
                            stream));
     HIP_TRY(hipMemcpyAsync(d_tdesc_, td.data(), sizeof(TopkDesc) * na, hipMemcpyHostToDevice,
                            stream));
+    /* work counters start clean every round (a failed round leaves no
+     * stale counts behind) */
+    HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
+    wq_parity_[0] = wq_parity_[1] = 0;
     HIP_TRY(hipEventRecord(ev0_, stream));
     for (int f = 0; f < 2; f++) {
         if (wg[f] == 0) continue;
@@ -414,12 +423,30 @@ int Engine::round(std::vector<ScanItem *> &finished) {
         m.xprefix = d_map_[f];
         m.xstart = d_map_[f] + 8 * (n + 1);
         m.n = (int32_t)n;
-        if (f == 0)
-            k_count_f4_batch<GEPI_FFUNC_ENTROPY><<<(unsigned)wg[f], F4_THREADS, 0, stream>>>(
-                d_bdesc_[f], m);
-        else
-            k_count_f4_batch<GEPI_FFUNC_FISHERPV><<<(unsigned)wg[f], F4_THREADS, 0, stream>>>(
-                d_bdesc_[f], m);
+        m.wq = d_wq_ + 16 * f;
+        m.wq_parity = wq_parity_[f];
+        const int per_cu = f4_wg_per_cu(2);
+        /* one slot per workgroup: the grid covers the largest XCD share */
+        int64_t xmax = 0;
+        for (int x = 0; x < 8; x++) xmax = std::max<int64_t>(xmax, map[f][(size_t)x * (n + 1) + n]);
+        unsigned grid = (unsigned)(8 * xmax);
+        if (per_cu > 0) {
+            wq_parity_[f] ^= 1;
+            grid = (unsigned)std::min<int64_t>(
+                std::max<int64_t>(8, (per_cu * (int64_t)device_cus(device)) / 8 * 8),
+                ((wg[f] + 7) / 8) * 8);
+        }
+        if (f == 0) {
+            if (per_cu > 0)
+                k_count_f4_batch<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+            else
+                k_count_f4_batch<GEPI_FFUNC_ENTROPY, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+        } else {
+            if (per_cu > 0)
+                k_count_f4_batch<GEPI_FFUNC_FISHERPV, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+            else
+                k_count_f4_batch<GEPI_FFUNC_FISHERPV, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+        }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev1_, stream));

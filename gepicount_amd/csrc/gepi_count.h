@@ -473,6 +473,239 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
     stage_flush(g, stage);
 }
 
+/* Persistent fp4 count loop (batched scans, whose tiles are short and come
+ * from many scans).  A launch has a few workgroups per CU; each takes tiles
+ * from the work counter of its XCD (workgroups are
+ * dispatched to the 8 XCDs round robin, XCD = blockIdx.x % 8) until that
+ * XCD's share of the band is done.  Per tile: the super-step pipeline into
+ * the class accumulators, then the fused filter; the first super-step of
+ * the next tile is fetched before the filter runs, so its latency is
+ * hidden.  Survivors stay staged in LDS across tiles of the same scan
+ * until the stage is half full.
+ *
+ * Src provides grab(n) (thread 0 only; n = the workgroup's earlier grabs:
+ * its next job, ok = 0 when there is none) and args(item) (the scan's count
+ * arguments).
+ * Work counters come in two sets of 8 used by alternate launches; each
+ * launch zeroes the other set for the next one (no memset, capture safe).  */
+struct F4Job {
+    int64_t tile;
+    int32_t item;
+    int32_t ok;
+};
+
+/* A job read back from LDS by every lane, made wave-uniform (scalar
+ * registers) for the compiler. */
+__device__ __forceinline__ F4Job uniform_job(const F4Job &j) {
+    F4Job o;
+    const uint64_t t = (uint64_t)j.tile;
+    o.tile = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(t >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)t));
+    o.item = __builtin_amdgcn_readfirstlane(j.item);
+    o.ok = __builtin_amdgcn_readfirstlane(j.ok);
+    return o;
+}
+
+template <int FFUN, class Src>
+__device__ __forceinline__ void count_f4_run(const Src &src) {
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_ROWS * F4_PITCH];
+    __shared__ PendStage stage;
+    __shared__ F4Job s_job[2];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wa = wave >> 1, wb = wave & 1;
+    const int l16 = lane & 15, lg = lane >> 4;
+
+    /* Staging.  A super-step of a tile is 288 plane rows of 64 bytes (4
+     * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
+     * contiguous in the layout.  uint4 i < 768 of the A rows and i < 384 of
+     * the B rows; thread t takes A i = t + 256j (j < 3) and B i = t, t + 256
+     * (t < 128), i.e. row i/4, column t%4.                                  */
+    const bool hasB1 = tid < 128;
+    const int dA = (tid >> 2) * F4_PITCH + (tid & 3);
+    const int dB = (F4_RA + (tid >> 2)) * F4_PITCH + (tid & 3);
+    constexpr int D64 = 64 * F4_PITCH; /* 64 rows further down */
+    /* lane's fragment rows: A (i, p) is row (32wa + 16i + l16)*3 + p, B q is
+     * row 192 + (16wb + l16)*3 + q; the lane takes 16-byte column lg.      */
+    const int fA = (32 * wa + l16) * 3 * F4_PITCH + lg;
+    const int fB = (F4_RA + (16 * wb + l16) * 3) * F4_PITCH + lg;
+
+    /* geometry of one tile */
+    struct Geo {
+        const uint4 *gA, *gB; /* this thread's first staging row of super-step 0 */
+        int64_t ustride;
+        int32_t ta, tb, u1, n_u, last, ks1;
+    };
+    auto geo_of = [&](const CountArgs &g, int64_t tile) {
+        Geo o;
+        tile_of_f4(tile, o.ta, o.tb);
+        const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
+        o.ustride = (int64_t)g.n_var_pad * 3 * 4; /* uint4 per super-step */
+        o.gA = base + (int64_t)o.ta * (F4_RA * 4) + tid;
+        o.gB = base + (int64_t)o.tb * (TB4 * 3 * 4) + tid;
+        o.ks1 = g.n_ks - g.ks0;
+        o.u1 = f4_super(g.ks0);
+        o.n_u = o.u1 + f4_super(o.ks1);
+        o.last = o.n_u > 0 ? o.n_u - 1 : 0; /* n_u == 0: no K loop, super-step 0 exists */
+        return o;
+    };
+    uint4 pa0, pa1, pa2, pb0, pb1 = make_uint4(0u, 0u, 0u, 0u);
+    auto fetch = [&](const Geo &o, int32_t u) {
+        const int64_t off = (int64_t)(u < o.last ? u : o.last) * o.ustride;
+        pa0 = o.gA[off];
+        pa1 = o.gA[off + 256];
+        pa2 = o.gA[off + 512];
+        pb0 = o.gB[off];
+        if (hasB1) pb1 = o.gB[off + 256];
+    };
+    auto put = [&](uint4 *W) {
+        W[dA] = pa0;
+        W[dA + D64] = pa1;
+        W[dA + 2 * D64] = pa2;
+        W[dB] = pb0;
+        if (hasB1) W[dB + D64] = pb1;
+    };
+
+    if (tid == 0) {
+        s_job[0] = src.grab(0);
+        stage.n = 0;
+    }
+    __syncthreads();
+    F4Job job = uniform_job(s_job[0]);
+    if (!job.ok) return;
+    CountArgs g = src.args(job.item);
+    Geo geo = geo_of(g, job.tile);
+    fetch(geo, 0);
+    put(lds[0]);
+    fetch(geo, 1);
+    if (tid == 0) s_job[1] = src.grab(1);
+    __syncthreads();
+    if (geo.n_u == 0) __syncthreads();
+
+    for (int32_t t = 0;; t++) {
+        /* super-step u into one class's accumulators, its first np K steps
+         * (the rest of a class's last super-step is padding); the staging
+         * pipeline runs on across the two phases.                          */
+        auto step = [&](v4f (&acc)[2][3][3], int32_t u, int32_t np) {
+            const uint4 *L = lds[u & 1];
+            uint4 wA[2][3], wB[3];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
+#pragma unroll
+            for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
+            /* stage super-step u+1 (a harmless extra copy on the last one:
+             * that buffer is not read again), prefetch u+2 */
+            put(lds[(u & 1) ^ 1]);
+            fetch(geo, u + 2);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (k < np) {
+                    const uint32_t m = 0x11111111u << k;
+                    v8i b[3];
+#pragma unroll
+                    for (int q = 0; q < 3; q++) b[q] = frag_f4(wB[q], m);
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int p = 0; p < 3; p++) {
+                            const v8i a = frag_f4(wA[i][p], m);
+#pragma unroll
+                            for (int q = 0; q < 3; q++)
+                                acc[i][p][q] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                                    a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, 128 - k, 0,
+                                    128 - k);
+                        }
+                }
+            }
+            __syncthreads();
+        };
+        v4f acc0[2][3][3], acc1[2][3][3];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc0[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int32_t u = 0; u < geo.u1; u++) {
+            const int32_t r = g.ks0 - 3 * u;
+            step(acc0, u, r < 3 ? r : 3);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc1[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int32_t u = geo.u1; u < geo.n_u; u++) {
+            const int32_t r = geo.ks1 - 3 * (u - geo.u1);
+            step(acc1, u, r < 3 ? r : 3);
+        }
+
+        /* the filter's constants are read before the next tile's loads are
+         * issued: waits on global loads are in issue order */
+        const FilterConsts fc = filter_consts(g.st);
+        /* next job (written before this tile's first barrier) and its first
+         * super-step, in flight during the filter */
+        const F4Job nj = uniform_job(s_job[(t + 1) & 1]);
+        CountArgs gn;
+        Geo geon;
+        if (nj.ok) {
+            gn = src.args(nj.item);
+            geon = geo_of(gn, nj.tile);
+            fetch(geon, 0);
+        }
+
+        /* filter: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
+         * variant 32wa + 16i + 4lg + r against B variant 16wb + l16.       */
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int32_t A = geo.ta * TM + 32 * wa + 16 * i + 4 * lg + r;
+                const int32_t B = geo.tb * TB4 + 16 * wb + l16;
+                if (A >= g.n_var || B > A) continue;
+                int32_t c[2][3][3];
+#pragma unroll
+                for (int k = 0; k < 2; k++)
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+#pragma unroll
+                        for (int q = 0; q < 3; q++)
+                            c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
+                filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+            }
+        }
+        if (!nj.ok) break;
+        /* the staged survivors go out when the scan changes or the stage is
+         * half full (thread 0 may clamp stage.n while others read it here;
+         * both values give the same answer)                               */
+        __syncthreads();
+        const bool flush = nj.item != job.item || stage.n >= (uint32_t)(PEND_LDS / 2);
+        if (flush) stage_flush_after_barrier(g, stage);
+        put(lds[0]);
+        fetch(geon, 1);
+        __syncthreads(); /* super-step 0 staged; the flush's copies are done */
+        if (tid == 0) {
+            if (flush) stage.n = 0;
+            s_job[t & 1] = src.grab(t + 2);
+        }
+        job = nj;
+        g = gn;
+        geo = geon;
+        if (geo.n_u == 0) __syncthreads();
+    }
+    stage_flush(g, stage);
+}
+
+/* Work counters of a launch: set (parity) of two sets of 8. */
+__device__ __forceinline__ void zero_other_counters(uint32_t *wq, int32_t parity) {
+    if (blockIdx.x == 0 && threadIdx.x < 8) wq[8 * (parity ^ 1) + threadIdx.x] = 0u;
+}
+
+/* One tile of the band per workgroup (tiles as band_tile maps them). */
 template <int FFUN>
 __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
     count_f4_tile<FFUN>(g, band_tile(g.tile_begin, g.n_tiles));

@@ -143,8 +143,9 @@ __device__ __forceinline__ void stage_init(PendStage &ps) {
     if (threadIdx.x == 0) ps.n = 0;
 }
 
-__device__ __forceinline__ void stage_flush(const CountArgs &g, PendStage &ps) {
-    __syncthreads();
+/* The flush proper: every push is complete (the caller has passed a
+ * barrier after the last push).                                           */
+__device__ __forceinline__ void stage_flush_after_barrier(const CountArgs &g, PendStage &ps) {
     if (threadIdx.x == 0) {
         const uint32_t n = ps.n < (uint32_t)PEND_LDS ? ps.n : (uint32_t)PEND_LDS;
         ps.n = n;
@@ -159,6 +160,11 @@ __device__ __forceinline__ void stage_flush(const CountArgs &g, PendStage &ps) {
         else
             atomicOr(&g.st->pend_overflow, 1u);
     }
+}
+
+__device__ __forceinline__ void stage_flush(const CountArgs &g, PendStage &ps) {
+    __syncthreads();
+    stage_flush_after_barrier(g, ps);
 }
 
 /* Per-scan filter constants, read once per workgroup (filter_consts). */

@@ -97,6 +97,8 @@ class Engine {
     gepi::ItemReport *d_rep_ = nullptr;
     size_t desc_cap_ = 0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    uint32_t *d_wq_ = nullptr; /* fp4 work counters: [objective][2 sets][8] */
+    int32_t wq_parity_[2] = {0, 0};
 };
 
 } // namespace gepi_host

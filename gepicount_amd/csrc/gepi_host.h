@@ -75,6 +75,13 @@ namespace gepi_host {
 /* Makes `device` current; fails unless it is a gfx950. */
 int set_device(int device);
 
+/* Compute units of a device seen by set_device (256 if unknown). */
+int32_t device_cus(int device);
+
+/* Workgroups per CU of the persistent batched fp4 count launch (0: one
+ * tile per workgroup); dflt unless GEPI_F4_WG_PER_CU overrides it. */
+int f4_wg_per_cu(int dflt);
+
 /* Grows the t*log2(t) and ln(t!) tables to cover t <= tmax. */
 int ensure_tables(DeviceTables &tabs, int32_t tmax);
 

@@ -69,6 +69,22 @@ extern "C" const char *gepi_version(void) { return GEPI_VERSION; }
 
 namespace gepi_host {
 
+/* compute units per device, recorded by set_device */
+static int32_t g_cus[64];
+
+int32_t device_cus(int device) {
+    return device >= 0 && device < 64 && g_cus[device] > 0 ? g_cus[device] : 256;
+}
+
+int f4_wg_per_cu(int dflt) {
+    /* launch form of the batched fp4 count kernel: GEPI_F4_WG_PER_CU = k > 0
+     * runs k persistent workgroups per CU, 0 one tile per workgroup */
+    const char *v = getenv("GEPI_F4_WG_PER_CU");
+    if (!v) return dflt;
+    const int k = atoi(v);
+    return k < 0 ? 0 : k > 16 ? 16 : k;
+}
+
 int set_device(int device) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -83,6 +99,7 @@ int set_device(int device) {
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return gepi_fail(GEPI_ERR_DEVICE, "device %d is %s; this build targets gfx950", device,
                          prop.gcnArchName);
+    if (device < 64) g_cus[device] = prop.multiProcessorCount;
     return GEPI_OK;
 }
 
