@@ -68,7 +68,7 @@ struct F4BatchSrc {
         j.tile = d[lo].g.tile_begin + map.xstart[(int64_t)x * map.n + lo] + (slot - pre[lo]);
         return j;
     }
-    __device__ __forceinline__ CountArgs args(int32_t item) const { return d[item].g; }
+    __device__ __forceinline__ const CountArgs *argp(int32_t item) const { return &d[item].g; }
 };
 
 /* One (scan, tile) per workgroup: slot = blockIdx.x / 8 of XCD blockIdx.x % 8. */
@@ -96,7 +96,7 @@ struct F4BatchOnceSrc {
         j.tile = d[lo].g.tile_begin + map.xstart[(int64_t)x * map.n + lo] + (slot - pre[lo]);
         return j;
     }
-    __device__ __forceinline__ CountArgs args(int32_t item) const { return d[item].g; }
+    __device__ __forceinline__ const CountArgs *argp(int32_t item) const { return &d[item].g; }
 };
 
 /* PERSIST: workgroups take slots from the work counters (grid of a few

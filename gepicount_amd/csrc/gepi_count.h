@@ -484,8 +484,11 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
  * until the stage is half full.
  *
  * Src provides grab(n) (thread 0 only; n = the workgroup's earlier grabs:
- * its next job, ok = 0 when there is none) and args(item) (the scan's count
- * arguments).
+ * its next job, ok = 0 when there is none) and argp(item), the scan's count
+ * arguments in global memory.  The arguments are read through that pointer
+ * (uniform, into a kernel-argument table): the compiler then keeps them in
+ * scalar registers and combines a wave's atomic adds on the pending-list
+ * counter into one, which matters when many evaluations survive.
  * Work counters come in two sets of 8 used by alternate launches; each
  * launch zeroes the other set for the next one (no memset, capture safe).  */
 struct F4Job {
@@ -574,8 +577,8 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     __syncthreads();
     F4Job job = uniform_job(s_job[0]);
     if (!job.ok) return;
-    CountArgs g = src.args(job.item);
-    Geo geo = geo_of(g, job.tile);
+    const CountArgs *gp = src.argp(job.item);
+    Geo geo = geo_of(*gp, job.tile);
     fetch(geo, 0);
     put(lds[0]);
     fetch(geo, 1);
@@ -584,6 +587,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     if (geo.n_u == 0) __syncthreads();
 
     for (int32_t t = 0;; t++) {
+        const CountArgs &g = *gp;
         /* super-step u into one class's accumulators, its first np K steps
          * (the rest of a class's last super-step is padding); the staging
          * pipeline runs on across the two phases.                          */
@@ -650,11 +654,11 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         /* next job (written before this tile's first barrier) and its first
          * super-step, in flight during the filter */
         const F4Job nj = uniform_job(s_job[(t + 1) & 1]);
-        CountArgs gn;
+        const CountArgs *gpn = gp;
         Geo geon;
         if (nj.ok) {
-            gn = src.args(nj.item);
-            geon = geo_of(gn, nj.tile);
+            gpn = src.argp(nj.item);
+            geon = geo_of(*gpn, nj.tile);
             fetch(geon, 0);
         }
 
@@ -693,11 +697,11 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             s_job[t & 1] = src.grab(t + 2);
         }
         job = nj;
-        g = gn;
+        gp = gpn;
         geo = geon;
         if (geo.n_u == 0) __syncthreads();
     }
-    stage_flush(g, stage);
+    stage_flush(*gp, stage);
 }
 
 /* Work counters of a launch: set (parity) of two sets of 8. */

@@ -114,12 +114,25 @@ struct PendStage {
     unsigned long long base;
 };
 
+/* Atomics on the scan state through global-address-space pointers: the
+ * state pointer may come from a descriptor in memory, where the compiler
+ * cannot tell its address space, and global atomics on a uniform address
+ * let it combine a wave's atomics into one (generic ones it does not). */
+typedef __attribute__((address_space(1))) unsigned long long gu64_t;
+typedef __attribute__((address_space(1))) uint32_t gu32_t;
+__device__ __forceinline__ unsigned long long st_add(unsigned long long *p, unsigned long long v) {
+    return __hip_atomic_fetch_add((gu64_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_or(uint32_t *p, uint32_t v) {
+    __hip_atomic_fetch_or((gu32_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void push_global(const CountArgs &g, const Pend &p) {
-    unsigned long long pos = atomicAdd(&g.st->pend_count, 1ull);
+    unsigned long long pos = st_add(&g.st->pend_count, 1ull);
     if (pos < g.pcap)
         g.pend[pos] = p;
     else
-        atomicOr(&g.st->pend_overflow, 1u);
+        st_or(&g.st->pend_overflow, 1u);
 }
 
 __device__ __forceinline__ void push_pending(const CountArgs &g, PendStage &ps, int32_t w,
@@ -149,7 +162,7 @@ __device__ __forceinline__ void stage_flush_after_barrier(const CountArgs &g, Pe
     if (threadIdx.x == 0) {
         const uint32_t n = ps.n < (uint32_t)PEND_LDS ? ps.n : (uint32_t)PEND_LDS;
         ps.n = n;
-        ps.base = n ? atomicAdd(&g.st->pend_count, (unsigned long long)n) : 0ull;
+        ps.base = n ? st_add(&g.st->pend_count, (unsigned long long)n) : 0ull;
     }
     __syncthreads();
     const uint32_t n = ps.n;
@@ -158,7 +171,7 @@ __device__ __forceinline__ void stage_flush_after_barrier(const CountArgs &g, Pe
         if (pos < g.pcap)
             g.pend[pos] = ps.rec[i];
         else
-            atomicOr(&g.st->pend_overflow, 1u);
+            st_or(&g.st->pend_overflow, 1u);
     }
 }
 
