@@ -36,7 +36,7 @@ FP4_MFMA_PEAK_FLOPS = 10.1e15
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="large",
                     choices=["small", "medium", "large", "mix64", "serve"],
@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--topk", type=int, default=10000)
     ap.add_argument("--ffun", type=int, default=0)
     ap.add_argument("--count-path", type=int, default=0)
+    ap.add_argument("--cli", type=int, default=1,
+                    help="large/medium/small: also time the command line end to end")
     ap.add_argument("--cpu-seconds", type=float, default=25.0,
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -267,6 +269,34 @@ def run_serve(args):
     print(json.dumps(line))
 
 
+def cli_wall(p, args):
+    """End-to-end wall time of the gepicount command line on the same panel
+    written as a PLINK fileset: .bed/.bim/.fam reading, upload and decode,
+    QA report, scan and the printed top-k report (stderr, discarded)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    from gepicount_amd import synth
+    exe = os.path.join(ROOT, "gepicount_amd", "gepicount")
+    if not os.path.exists(exe):
+        return None
+    d = tempfile.mkdtemp(prefix="gepi_bench_")
+    try:
+        root = os.path.join(d, "panel")
+        synth.write_plink(root, p)
+        cmd = [exe, "-i", root, "-N", str(args.topk)] + (["-f"] if args.ffun else [])
+        t = time.perf_counter()
+        r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+        wall = time.perf_counter() - t
+        if r.returncode != 0:
+            return {"error": "exit status %d" % r.returncode}
+        return {"wall_s": wall, "command": "gepicount -i <prefix> -N %d%s" % (
+            args.topk, " -f" if args.ffun else "")}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -342,6 +372,7 @@ def main():
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(panel, m0, m1, n, args.cpu_seconds, args.topk, args.ffun)
     panel.close()
+    cli = cli_wall(p, args) if args.cli else None
 
     lat = sorted(times)
     line = {
@@ -357,6 +388,8 @@ def main():
                    "evals": evals, "topk": args.topk, "count_path": stats[-1]["count_path"]},
         "p50_ms": 1000.0 * lat[len(lat) // 2], "p99_ms": 1000.0 * lat[min(len(lat) - 1,
                                                                            int(0.99 * len(lat)))],
+        "median_ms": 1000.0 * statistics.median(times),
+        "cli": cli,
         "variant_pairs_per_s": (n_var * (n_var - 1) // 2) / statistics.mean(times),
         "exact_evals": stats[-1]["exact_evals"], "gen_s": gen_s,
         "roofline": roof, "cpu_baseline": cpu,

@@ -139,3 +139,47 @@ def test_batch_failing_and_degenerate_items_leave_others_intact():
                             it["topmost"], ffun=it["ffun"])
         assert res[i][2] == 0
         assert_same(res[i][0], ref, "item %d" % i)
+
+
+@pytest.mark.parametrize("per_cu", ["0", "1", "4"])
+def test_batch_launch_forms_agree(per_cu, monkeypatch):
+    """The batched count kernel's launch form (one tile per workgroup, or k
+    persistent workgroups per CU taking tiles from the work counters) does
+    not change any result."""
+    print("case batch launch form per_cu=%s" % per_cu, flush=True)
+    panels, items = make_mix(11, 10, vmax=400)
+    monkeypatch.setenv("GEPI_F4_WG_PER_CU", per_cu)
+    with ge.Engine() as eng:
+        res, bstats, alone = run_mix(panels, items, eng)
+    assert bstats["items_failed"] == 0
+    for i, (got, st, status) in enumerate(res):
+        assert status == 0, (i, status)
+        assert_same(got, alone[i], "item %d (per_cu %s) vs alone" % (i, per_cu))
+
+
+def test_batch_everything_survives_early():
+    """Large top-k on small panels: the first bands admit every evaluation
+    (no threshold yet), so staged survivors overflow the LDS stage and the
+    pending list takes them one by one."""
+    print("case batch heavy survivors", flush=True)
+    rng = np.random.default_rng(21)
+    panels, items = [], []
+    for j in range(4):
+        n_var, n = int(rng.integers(150, 260)), int(rng.integers(300, 900))
+        codes = orc.random_codes(rng, n_var, n, 0.01)
+        bits, valid = orc.planes_from_codes(codes)
+        panels.append((bits, valid, n))
+        m0, m1 = orc.masks_from_classes(np.where(rng.random(n) < 0.4, 1, 0))
+        R = bits.shape[0]
+        items.append(dict(panel_index=j, mask_ctrl=m0, mask_case=m1,
+                          topmost=int(min(5 * R * (R + 1) // 2, 60000)), ffun=0))
+    with ge.Engine() as eng:
+        res, bstats, alone = run_mix(panels, items, eng)
+    assert bstats["items_failed"] == 0
+    for i, (got, st, status) in enumerate(res):
+        assert status == 0, (i, status)
+        assert_same(got, alone[i], "item %d vs alone" % i)
+        bits, valid, n = panels[items[i]["panel_index"]]
+        ref = orc.gold_scan(orc.Panel(bits, valid, items[i]["mask_ctrl"], items[i]["mask_case"], n),
+                            items[i]["topmost"])
+        assert_same(got, ref, "item %d vs oracle" % i)
