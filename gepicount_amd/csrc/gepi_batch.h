@@ -148,6 +148,8 @@ struct TopkDesc {
     ItemReport *report;
     uint32_t k;
     int32_t ffun;
+    int32_t n_a4;             /* A blocks of the scan (fp4 tile numbering)    */
+    int32_t pad;
 };
 
 constexpr int TOPK_THREADS = 1024;
@@ -296,7 +298,7 @@ __global__ void __launch_bounds__(TOPK_THREADS) k_topk_wg(const TopkDesc *__rest
                 if (keep && cand_ovf) {
                     const int64_t a = (int64_t)(c.idx >> 32) >> 1;
                     const int64_t b = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull) >> 1;
-                    const int64_t tl = tile_index(a, b, TM, 1);
+                    const int64_t tl = tile_index(a, b, TM, d.n_a4);
                     keep = tl < d.drop_t0 || tl >= d.drop_t1;
                 }
             }

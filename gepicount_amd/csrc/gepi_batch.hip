@@ -359,6 +359,8 @@ int Engine::round(std::vector<ScanItem *> &finished) {
         t.report = d_rep_ + i;
         t.k = (uint32_t)it->k;
         t.ffun = it->ffun;
+        t.n_a4 = (int32_t)(it->n_var_pad / TM);
+        t.pad = 0;
     }
     /* per objective: descriptors with the longest K loops first (their
      * workgroups are dispatched first; short ones fill the tail), and the

@@ -311,17 +311,6 @@ constexpr int F4_FMT = 4;       /* operand format code of e2m1                  
  * starts on a super-step, class 0 at 0 and class 1 at f4_super(ks0).       */
 __host__ __device__ inline int32_t f4_super(int32_t ks) { return (ks + 2) / 3; }
 
-/* Workgroup tile t = ta*(ta+1) + tb, 0 <= tb <= 2*ta+1: A variants
- * [64ta, 64ta+64) against B variants [32tb, 32tb+32). */
-__host__ __device__ inline void tile_of_f4(int64_t t, int32_t &ta, int32_t &tb) {
-    double r = sqrt(4.0 * (double)t + 1.0);
-    int64_t a = (int64_t)((r - 1.0) * 0.5);
-    while (a * (a + 1) > t) a--;
-    while ((a + 1) * (a + 2) <= t) a++;
-    ta = (int32_t)a;
-    tb = (int32_t)(t - a * (a + 1));
-}
-
 /* One lane's fp4 fragment of K step p of a super-step: the lane's four
  * layout dwords masked to nibble bit p.  A set bit is e2m1 0.5 * 2^p; with
  * block scale 2^(1-p) on both operands every product of two set bits is
@@ -340,7 +329,7 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
     __shared__ PendStage stage;
     stage_init(stage);
     int32_t ta, tb;
-    tile_of_f4(tile, ta, tb);
+    tile_of_f4(tile, g.n_var_pad / TM, ta, tb);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -542,7 +531,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     };
     auto geo_of = [&](const CountArgs &g, int64_t tile) {
         Geo o;
-        tile_of_f4(tile, o.ta, o.tb);
+        tile_of_f4(tile, g.n_var_pad / TM, o.ta, o.tb);
         const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
         o.ustride = (int64_t)g.n_var_pad * 3 * 4; /* uint4 per super-step */
         o.gA = base + (int64_t)o.ta * (F4_RA * 4) + tid;

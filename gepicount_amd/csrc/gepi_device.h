@@ -81,21 +81,63 @@ __host__ __device__ inline void tile_of(int64_t t, int32_t &ta, int32_t &tb) {
     tb = (int32_t)(t - a * (a + 1) / 2);
 }
 
+/* Workgroup tile t of a scan with n_a A blocks: A variants [64ta, 64ta+64)
+ * against B variants [32tb, 32tb+32), 0 <= tb <= 2ta+1 (row ta holds 2ta+2
+ * tiles, so n_a(n_a+1) in all).  Tiles are numbered in panels of
+ * F4_PANEL rows; panel rows [a0, a0+P) take indices [a0(a0+1),
+ * (a0+P)(a0+P+1)), first the rectangle tb < 2a0+2 that all of them share,
+ * column by column, then the rest of each row (row a0+i has 2i more tiles)
+ * in row order.  Workgroups running together on one XCD take consecutive
+ * tiles, so they share 8 A blocks and 8 B blocks in that XCD's L2; in row
+ * order they shared one A block and read 64 B blocks that the next row
+ * needs again only after the whole row has gone through.                 */
+constexpr int F4_PANEL = 8;
+__host__ __device__ inline void tile_of_f4(int64_t t, int32_t n_a, int32_t &ta, int32_t &tb) {
+    double r = sqrt(4.0 * (double)t + 1.0);
+    int64_t a = (int64_t)((r - 1.0) * 0.5);
+    while (a * (a + 1) > t) a--;
+    while ((a + 1) * (a + 2) <= t) a++;
+    const int64_t a0 = a - a % F4_PANEL;
+    const int64_t P = n_a - a0 < F4_PANEL ? n_a - a0 : F4_PANEL;
+    const int64_t W = 2 * a0 + 2;
+    int64_t l = t - a0 * (a0 + 1);
+    if (l < P * W) {
+        if (P == F4_PANEL) {
+            tb = (int32_t)(l >> 3);
+            ta = (int32_t)(a0 + (l & 7));
+        } else {
+            tb = (int32_t)(l / P);
+            ta = (int32_t)(a0 + l % P);
+        }
+    } else {
+        l -= P * W; /* i(i-1) tiles of the rest precede row a0+i */
+        int64_t i = 1;
+        while ((i + 1) * i <= l) i++;
+        ta = (int32_t)(a0 + i);
+        tb = (int32_t)(W + l - i * (i - 1));
+    }
+}
+
 /* Linear workgroup tile of variant pair (a >= b).  Square tiles of side tv
  * (popcount, int8 MFMA): ta*(ta+1)/2 + tb.  fp4 tiles of 64 x 32 variants
- * (rect != 0): ta*(ta+1) + tb with ta = a/64, tb = b/32.                   */
-__host__ __device__ inline int64_t tile_index(int64_t a, int64_t b, int32_t tv, int32_t rect) {
-    if (rect) {
+ * (n_a4 > 0, the scan's A blocks): the inverse of tile_of_f4 at ta = a/64,
+ * tb = b/32.                                                              */
+__host__ __device__ inline int64_t tile_index(int64_t a, int64_t b, int32_t tv, int32_t n_a4) {
+    if (n_a4 > 0) {
         const int64_t ta = a / 64, tb = b / 32;
-        return ta * (ta + 1) + tb;
+        const int64_t a0 = ta - ta % F4_PANEL;
+        const int64_t P = n_a4 - a0 < F4_PANEL ? n_a4 - a0 : F4_PANEL;
+        const int64_t W = 2 * a0 + 2, i = ta - a0;
+        return a0 * (a0 + 1) + (tb < W ? tb * P + i : P * W + i * (i - 1) + (tb - W));
     }
     const int64_t ta = a / tv, tb = b / tv;
     return ta * (ta + 1) / 2 + tb;
 }
 
 /* Workgroup -> tile of the band.  Consecutive workgroups are dispatched to
- * the 8 XCDs round robin; each XCD gets a contiguous run of tiles so that
- * tiles sharing a row panel share that XCD's L2.                            */
+ * the 8 XCDs round robin; each XCD gets a contiguous run of tiles, so that
+ * the tiles running together on it are neighbours in the tile order and
+ * share panel rows in that XCD's L2.                                      */
 __device__ __forceinline__ int64_t band_tile(int64_t tile_begin, int32_t n_tiles) {
     const int32_t gidx = blockIdx.x;
     const int32_t xcd = gidx & 7, slot = gidx >> 3;

@@ -486,7 +486,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             HIP_TRY(hipMemcpyAsync(p->d_state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
             launch_select(p, p->d_cand[cur], (uint32_t)k, kept);
             k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
-                                                   t0, t1, tv, path == 3 ? 1 : 0);
+                                                   t0, t1, tv, path == 3 ? (int32_t)n_a4 : 0);
             k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
             HIP_TRY(hipGetLastError());
             cur ^= 1;
@@ -501,7 +501,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         if (hs.count > (unsigned long long)k) {
             launch_select(p, p->d_cand[cur], (uint32_t)k, hs.count);
             k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
-                                                   0, 0, tv, path == 3 ? 1 : 0);
+                                                   0, 0, tv, path == 3 ? (int32_t)n_a4 : 0);
             k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
             HIP_TRY(hipGetLastError());
             cur ^= 1;

@@ -193,10 +193,10 @@ static __global__ void __launch_bounds__(1024) k_sel_small(ScanState *st, const 
 }
 
 /* Keeps records with key <= the selected key; records of workgroup tiles in
- * [drop_t0, drop_t1) (geometry tv, rect as in tile_index) are dropped too.  */
+ * [drop_t0, drop_t1) (geometry tv, n_a4 as in tile_index) are dropped too. */
 static __global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__ src, ScanState *st,
                                                   Cand *__restrict__ dst, int64_t drop_t0,
-                                                  int64_t drop_t1, int32_t tv, int32_t rect) {
+                                                  int64_t drop_t1, int32_t tv, int32_t n_a4) {
     const uint64_t khi = st->key_hi, klo = st->key_lo;
     const unsigned long long n = st->count;
     for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -207,7 +207,7 @@ static __global__ void __launch_bounds__(256) k_compact(const Cand *__restrict__
         if (keep && drop_t1 > drop_t0) {
             const int64_t a = (int64_t)(c.idx >> 32) >> 1;
             const int64_t b = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull) >> 1;
-            const int64_t t = tile_index(a, b, tv, rect);
+            const int64_t t = tile_index(a, b, tv, n_a4);
             keep = t < drop_t0 || t >= drop_t1;
         }
         if (keep) {
