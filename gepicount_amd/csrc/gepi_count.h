@@ -322,8 +322,10 @@ __device__ __forceinline__ v8i frag_f4(const uint4 &w, uint32_t m) {
 }
 
 /* One 64 x 32-variant workgroup tile of the fp4 path (tile numbering as in
- * tile_of_f4).  Called by every thread of a 256-thread workgroup.          */
-template <int FFUN>
+ * tile_of_f4).  Called by every thread of a 256-thread workgroup.  F32: the
+ * filter works on the fp32 accumulators directly (the scan's class sizes
+ * have n0 n1 <= 2^24), else on their int32 conversions.                    */
+template <int FFUN, bool F32>
 __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_ROWS * F4_PITCH];
     __shared__ PendStage stage;
@@ -391,10 +393,6 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
             for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
 #pragma unroll
         for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
-        /* stage super-step u+1 (a harmless extra copy on the last one: that
-         * buffer is not read again), prefetch u+2 */
-        put(lds[(u & 1) ^ 1]);
-        fetch(u + 2 <= last ? u + 2 : last);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             if (k < np) {
@@ -414,6 +412,13 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
                     }
             }
         }
+        /* stage super-step u+1 (a harmless extra copy on the last one: that
+         * buffer is not read again), prefetch u+2.  After the MFMAs, so that
+         * a wave waiting for its loads has its MFMAs already issued; the
+         * other buffer's last reads were in super-step u-1, before the
+         * barrier that ended it.                                           */
+        put(lds[(u & 1) ^ 1]);
+        fetch(u + 2 <= last ? u + 2 : last);
         __syncthreads();
     };
     v4f acc0[2][3][3], acc1[2][3][3];
@@ -448,15 +453,27 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
             const int32_t A = ta * TM + 32 * wa + 16 * i + 4 * lg + r;
             const int32_t B = tb * TB4 + 16 * wb + l16;
             if (A >= g.n_var || B > A) continue;
-            int32_t c[2][3][3];
+            if constexpr (F32) {
+                float c[2][3][3];
 #pragma unroll
-            for (int k = 0; k < 2; k++)
+                for (int k = 0; k < 2; k++)
 #pragma unroll
-                for (int p = 0; p < 3; p++)
+                    for (int p = 0; p < 3; p++)
 #pragma unroll
-                    for (int q = 0; q < 3; q++)
-                        c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
-            filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+                        for (int q = 0; q < 3; q++)
+                            c[k][p][q] = k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r];
+                filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+            } else {
+                int32_t c[2][3][3];
+#pragma unroll
+                for (int k = 0; k < 2; k++)
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+#pragma unroll
+                        for (int q = 0; q < 3; q++)
+                            c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
+                filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+            }
         }
     }
     stage_flush(g, stage);
@@ -699,9 +716,9 @@ __device__ __forceinline__ void zero_other_counters(uint32_t *wq, int32_t parity
 }
 
 /* One tile of the band per workgroup (tiles as band_tile maps them). */
-template <int FFUN>
+template <int FFUN, bool F32>
 __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
-    count_f4_tile<FFUN>(g, band_tile(g.tile_begin, g.n_tiles));
+    count_f4_tile<FFUN, F32>(g, band_tile(g.tile_begin, g.n_tiles));
 }
 
 } // namespace gepi

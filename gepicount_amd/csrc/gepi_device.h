@@ -100,22 +100,20 @@ __host__ __device__ inline void tile_of_f4(int64_t t, int32_t n_a, int32_t &ta, 
     const int64_t a0 = a - a % F4_PANEL;
     const int64_t P = n_a - a0 < F4_PANEL ? n_a - a0 : F4_PANEL;
     const int64_t W = 2 * a0 + 2;
-    int64_t l = t - a0 * (a0 + 1);
+    const int64_t l = t - a0 * (a0 + 1);
+    int64_t ra, rb;
     if (l < P * W) {
-        if (P == F4_PANEL) {
-            tb = (int32_t)(l >> 3);
-            ta = (int32_t)(a0 + (l & 7));
-        } else {
-            tb = (int32_t)(l / P);
-            ta = (int32_t)(a0 + l % P);
-        }
+        rb = P == F4_PANEL ? l >> 3 : l / P;
+        ra = a0 + (l - rb * P);
     } else {
-        l -= P * W; /* i(i-1) tiles of the rest precede row a0+i */
+        const int64_t e = l - P * W; /* i(i-1) tiles of the rest precede row a0+i */
         int64_t i = 1;
-        while ((i + 1) * i <= l) i++;
-        ta = (int32_t)(a0 + i);
-        tb = (int32_t)(W + l - i * (i - 1));
+        while ((i + 1) * i <= e) i++;
+        ra = a0 + i;
+        rb = W + e - i * (i - 1);
     }
+    ta = (int32_t)ra;
+    tb = (int32_t)rb;
 }
 
 /* Linear workgroup tile of variant pair (a >= b).  Square tiles of side tv
@@ -303,13 +301,15 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, con
  *     T = (-(thr + 1e-6) ln 10 - ln(4yz) / 2) / m   (y, z clamped to >= 1);
  * its survivors are then checked exactly against the log-factorial table.
  * The test is s = fd^2 - RN(K*y*z*(1 - 2^-16) * q) > bias with fd = float(D)
- * squared exactly inside one fma: the float error of the right side is
- * below 2^-21 relative and the left side is exact, so every table meeting
- * the exact bound passes; D == 0 (an independent table, I == 0) fails
- * whenever K > 0, and K == 0 (bias -1) admits everything.  D and xx are
- * linear in the three per-row-pair quantities (N11, NA, NB):
- * D_f = {Dn, Db-Dn, Da-Dn, Dn-Da-Db, Da+Db-2Dn} with Dq = q0*z - q1*y
- * (24-bit products, all counts < 2^16), and likewise xx_f from S = q0 + q1.
+ * squared exactly inside one fma.  fd is D exactly when |D| <= 2^24 (always
+ * so on the float path) and within 2^-24 relative otherwise; the right side
+ * is within 2^-21 relative; the 2^-16 shrink of K covers both, so every
+ * table meeting the exact bound passes.  D == 0 (an independent table,
+ * I == 0) fails whenever K > 0, and K == 0 (bias -1) admits everything.
+ * D and xx are linear in the three per-row-pair quantities (N11, NA, NB):
+ * up to sign, D_f = {Dn, Db-Dn, Da-Dn, (Da-Dn)+Db, (Da-Dn)+(Db-Dn)} with
+ * Dq = q0*z - q1*y (only D^2 enters the test), and xx_f likewise from
+ * S = q0 + q1.
  * A row pair is flagged when the largest of its five s exceeds the bias;
  * the bits of the functions are rebuilt, from the same expressions, only
  * for the lanes with a flagged row pair.
@@ -317,14 +317,17 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, con
  * ROLLED: a runtime loop over the four row pairs keeps the code of callers
  * with many pairs per thread small; its operands advance through rotating
  * registers (an indexed array would be placed in scratch memory).          */
+template <typename DT>
 struct RowTest {
     float kyz, mf, bias;
-    /* s of the five functions of one row pair */
-    __device__ __forceinline__ void s5(int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db,
-                                       float Sb, float (&s)[5]) const {
-        const int32_t Dab = Da + Db;
+    /* s of the five functions of one row pair.  Every D below is one
+     * function's w z - x y up to sign, so |D| <= y z: exact for int32 and,
+     * when y z <= 2^24, for float.                                         */
+    __device__ __forceinline__ void s5(DT Dn, float Sn, DT Da, float Sa, DT Db, float Sb,
+                                       float (&s)[5]) const {
+        const DT Dnb = Db - Dn, Dna = Da - Dn;
+        const DT D[5] = {Dn, Dnb, Dna, Dna + Db, Dna + Dnb};
         const float Sab = Sa + Sb;
-        const int32_t D[5] = {Dn, Db - Dn, Da - Dn, Dn - Dab, Dab - 2 * Dn};
         const float X[5] = {Sn, Sb - Sn, Sa - Sn, (mf - Sab) + Sn, Sab - 2.0f * Sn};
 #pragma unroll
         for (int f = 0; f < 5; f++) {
@@ -333,14 +336,12 @@ struct RowTest {
             s[f] = __builtin_fmaf(fd, fd, -r);
         }
     }
-    __device__ __forceinline__ bool any(int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db,
-                                        float Sb) const {
+    __device__ __forceinline__ bool any(DT Dn, float Sn, DT Da, float Sa, DT Db, float Sb) const {
         float s[5];
         s5(Dn, Sn, Da, Sa, Db, Sb, s);
         return fmaxf(fmaxf(fmaxf(s[0], s[1]), fmaxf(s[2], s[3])), s[4]) > bias;
     }
-    __device__ __forceinline__ uint32_t bits(int32_t Dn, float Sn, int32_t Da, float Sa, int32_t Db,
-                                             float Sb) const {
+    __device__ __forceinline__ uint32_t bits(DT Dn, float Sn, DT Da, float Sa, DT Db, float Sb) const {
         float s[5];
         s5(Dn, Sn, Da, Sa, Db, Sb, s);
         uint32_t b = 0;
@@ -350,11 +351,27 @@ struct RowTest {
     }
 };
 
-template <int FFUN, bool ROLLED>
+/* Count arithmetic of the filter.  int32 counts (< 2^16): D = q0 z - q1 y
+ * with 24-bit products.  float counts (the fp4 path's accumulators, exact
+ * integers): when n0 n1 <= 2^24 every product q0 z, q1 y is at most n0 n1
+ * (a count of class c is at most n_c) and so exact, and fma gives D exactly
+ * with no conversion.                                                     */
+__device__ __forceinline__ int32_t dq_of(int32_t q0, int32_t q1, int32_t y, int32_t z) {
+    return __mul24(q0, z) - __mul24(q1, y);
+}
+__device__ __forceinline__ float dq_of(float q0, float q1, float y, float z) {
+    return __builtin_fmaf(q0, z, -(q1 * y));
+}
+__device__ __forceinline__ float sq_of(int32_t q0, int32_t q1) { return (float)(q0 + q1); }
+__device__ __forceinline__ float sq_of(float q0, float q1) { return q0 + q1; }
+__device__ __forceinline__ float yz_of(int32_t y, int32_t z) { return (float)__mul24(y, z); }
+__device__ __forceinline__ float yz_of(float y, float z) { return y * z; }
+
+template <int FFUN, bool ROLLED, typename CT>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
-                                            int32_t A, int32_t B, const int32_t (&c)[2][3][3]) {
-    const int32_t y = c[0][2][2], z = c[1][2][2];
-    const int32_t m = y + z;
+                                            int32_t A, int32_t B, const CT (&c)[2][3][3]) {
+    const CT y = c[0][2][2], z = c[1][2][2];
+    const CT m = y + z;
     const float mf = (float)m;
     float K = fc.kf;
     if (FFUN == GEPI_FFUNC_FISHERPV) {
@@ -365,22 +382,22 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
             if (T > 0.0f) K = expm1f(T) * (1.0f - 1e-5f);
         }
     }
-    RowTest rt;
-    rt.kyz = K * (float)__mul24(y, z) * (1.0f - 1.0f / 65536.0f);
+    RowTest<CT> rt;
+    rt.kyz = K * yz_of(y, z) * (1.0f - 1.0f / 65536.0f);
     rt.mf = mf;
     rt.bias = K == 0.0f ? -1.0f : 0.0f;
-    auto dq = [&](int32_t q0, int32_t q1) { return __mul24(q0, z) - __mul24(q1, y); };
-    auto sq = [&](int32_t q0, int32_t q1) { return (float)(q0 + q1); };
-    const int32_t Da0 = dq(c[0][0][2], c[1][0][2]), Da1 = dq(c[0][1][2], c[1][1][2]);
-    const int32_t Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
+    auto dq = [&](CT q0, CT q1) { return dq_of(q0, q1, y, z); };
+    auto sq = [&](CT q0, CT q1) { return sq_of(q0, q1); };
+    const CT Da0 = dq(c[0][0][2], c[1][0][2]), Da1 = dq(c[0][1][2], c[1][1][2]);
+    const CT Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
     const float Sa0 = sq(c[0][0][2], c[1][0][2]), Sa1 = sq(c[0][1][2], c[1][1][2]);
     const float Sb0 = sq(c[0][2][0], c[1][2][0]), Sb1 = sq(c[0][2][1], c[1][2][1]);
     /* row pair rp = 2ia + ib flagged in bit rp */
     uint32_t rows = 0;
     if (ROLLED) {
-        int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
-        int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
-        int32_t Da = Da0, Db = Db0, Dbo = Db1;
+        CT n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
+        CT n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
+        CT Da = Da0, Db = Db0, Dbo = Db1;
         float Sa = Sa0, Sb = Sb0, Sbo = Sb1;
 #pragma unroll 1
         for (int rp = 0; rp < 4; rp++) {
@@ -388,7 +405,7 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
             n0a = n0b; n0b = n0c; n0c = n0d;
             n1a = n1b; n1b = n1c; n1c = n1d;
             if (rp == 1) { Da = Da1; Sa = Sa1; }
-            int32_t t = Db; Db = Dbo; Dbo = t;
+            CT t = Db; Db = Dbo; Dbo = t;
             float u = Sb; Sb = Sbo; Sbo = u;
         }
     } else {
@@ -409,7 +426,14 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
             mask |= rt.bits(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) << 10;
         if (rows & 8u)
             mask |= rt.bits(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) << 15;
-        push_mask<FFUN>(g, ps, fc, A, B, c, mask);
+        int32_t ci[2][3][3];
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) ci[k][p][q] = (int32_t)c[k][p][q];
+        push_mask<FFUN>(g, ps, fc, A, B, ci, mask);
     }
 }
 

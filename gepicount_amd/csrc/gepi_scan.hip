@@ -428,6 +428,9 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     unsigned long long count = 0, exact_total = 0;
     int bands = 0;
     const bool entropy = par->ffun == GEPI_FFUNC_ENTROPY;
+    /* fp4: the filter runs on the fp32 accumulators when every product of a
+     * class-0 and a class-1 count is exact in float */
+    const bool f32 = (int64_t)n0 * n1 <= (int64_t)1 << 24;
     while (t0 < n_tiles) {
         const int64_t t1 = std::min<int64_t>(n_tiles, t0 + band);
         ca.tile_begin = t0;
@@ -437,10 +440,14 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         HIP_TRY(hipEventRecord(p->ev0, p->stream));
         const unsigned grid = (unsigned)(t1 - t0);
         if (path == 3) {
-            if (entropy)
-                k_count_f4<GEPI_FFUNC_ENTROPY><<<grid, F4_THREADS, 0, p->stream>>>(ca);
+            if (entropy && f32)
+                k_count_f4<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, p->stream>>>(ca);
+            else if (entropy)
+                k_count_f4<GEPI_FFUNC_ENTROPY, false><<<grid, F4_THREADS, 0, p->stream>>>(ca);
+            else if (f32)
+                k_count_f4<GEPI_FFUNC_FISHERPV, true><<<grid, F4_THREADS, 0, p->stream>>>(ca);
             else
-                k_count_f4<GEPI_FFUNC_FISHERPV><<<grid, F4_THREADS, 0, p->stream>>>(ca);
+                k_count_f4<GEPI_FFUNC_FISHERPV, false><<<grid, F4_THREADS, 0, p->stream>>>(ca);
         } else if (path == 2) {
             if (entropy)
                 k_count_mfma<GEPI_FFUNC_ENTROPY><<<grid, MF_THREADS, 0, p->stream>>>(ca);

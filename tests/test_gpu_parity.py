@@ -21,13 +21,33 @@ DATA = os.path.join(ROOT, "tests", "golden", "data")
 FIELDS = ("sA", "sB", "fun", "w", "x", "y", "z", "obj")
 
 
+def _keys(r):
+    return [(int(a), int(b), int(f)) for a, b, f in zip(r["sA"], r["sB"], r["fun"])]
+
+
+def mismatch_report(got, ref):
+    """Which evaluations are missing, extra or repeated in got, and got's
+    record for the reference's first differing evaluation."""
+    gk, rk = _keys(got), _keys(ref)
+    gs, rs = set(gk), set(rk)
+    dup = sorted({k for k in gk if gk.count(k) > 1})[:5]
+    lines = ["missing %s" % sorted(rs - gs)[:5], "extra %s" % sorted(gs - rs)[:5],
+             "repeated %s" % dup]
+    for i, k in enumerate(rk):
+        if i >= len(gk) or gk[i] != k or any(got[f][i] != ref[f][i] for f in FIELDS):
+            same = [got[j] for j in range(len(gk)) if gk[j] == k]
+            lines.append("ref rank %d %r; got for that evaluation %r" % (i, ref[i], same))
+            break
+    return "; ".join(lines)
+
+
 def assert_same(got, ref, label=""):
-    assert len(got) == len(ref), (label, len(got), len(ref))
+    assert len(got) == len(ref), (label, len(got), len(ref), mismatch_report(got, ref))
     for k in FIELDS:
         if not np.array_equal(got[k], ref[k]):
             bad = int(np.flatnonzero(got[k] != ref[k])[0])
-            raise AssertionError("%s: field %s differs first at rank %d: got %r ref %r" % (
-                label, k, bad, got[bad], ref[bad]))
+            raise AssertionError("%s: field %s differs first at rank %d: got %r ref %r; %s" % (
+                label, k, bad, got[bad], ref[bad], mismatch_report(got, ref)))
 
 
 def random_panel(seed, n_var, n, miss, cls_p=(0.5, 0.45, 0.05)):
@@ -83,6 +103,27 @@ def test_fisher_objective(seed, n_var, n, miss, k, path):
                               count_path=path)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), kk, ffun=1)
     assert_same(got, ref, "fisher")
+
+
+@pytest.mark.parametrize("ffun", [0, 1])
+@pytest.mark.parametrize("n0,n1", [(4096, 4096), (4200, 4200)])
+def test_fp4_filter_count_types(n0, n1, ffun):
+    """The fp4 path filters on its fp32 accumulators when every product of a
+    class-0 and a class-1 count is exact in float (n0 n1 <= 2^24; 4096 x 4096
+    is the largest balanced case, where products reach 2^24) and on int32
+    conversions above that (4200 x 4200).  Both must admit every evaluation
+    the exact top-k needs."""
+    print("case fp4 filter n0=%d n1=%d ffun=%d" % (n0, n1, ffun), flush=True)
+    rng = np.random.default_rng(83)
+    n = n0 + n1
+    codes = orc.random_codes(rng, 48, n, 0.02)
+    cls = rng.permutation(np.r_[np.zeros(n0, np.int64), np.ones(n1, np.int64)])
+    bits, valid = orc.planes_from_codes(codes)
+    m0, m1 = orc.masks_from_classes(cls)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=300, ffun=ffun, count_path=3)
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), 300, ffun=ffun)
+    assert st["n_class0"] == n0 and st["n_class1"] == n1
+    assert_same(got, ref, "fp4-filter")
 
 
 def test_kat_test_fixture_top5():
