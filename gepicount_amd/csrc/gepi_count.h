@@ -606,10 +606,6 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
                 for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
 #pragma unroll
             for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
-            /* stage super-step u+1 (a harmless extra copy on the last one:
-             * that buffer is not read again), prefetch u+2 */
-            put(lds[(u & 1) ^ 1]);
-            fetch(geo, u + 2);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 if (k < np) {
@@ -630,6 +626,11 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
                         }
                 }
             }
+            /* stage super-step u+1 (a harmless extra copy on the last one:
+             * that buffer is not read again), prefetch u+2; after the MFMAs,
+             * as in count_f4_tile */
+            put(lds[(u & 1) ^ 1]);
+            fetch(geo, u + 2);
             __syncthreads();
         };
         v4f acc0[2][3][3], acc1[2][3][3];
@@ -677,15 +678,28 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
                 const int32_t A = geo.ta * TM + 32 * wa + 16 * i + 4 * lg + r;
                 const int32_t B = geo.tb * TB4 + 16 * wb + l16;
                 if (A >= g.n_var || B > A) continue;
-                int32_t c[2][3][3];
+                if (g.f32) {
+                    float c[2][3][3];
 #pragma unroll
-                for (int k = 0; k < 2; k++)
+                    for (int k = 0; k < 2; k++)
 #pragma unroll
-                    for (int p = 0; p < 3; p++)
+                        for (int p = 0; p < 3; p++)
 #pragma unroll
-                        for (int q = 0; q < 3; q++)
-                            c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
-                filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+                            for (int q = 0; q < 3; q++)
+                                c[k][p][q] = k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r];
+                    filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+                } else {
+                    int32_t c[2][3][3];
+#pragma unroll
+                    for (int k = 0; k < 2; k++)
+#pragma unroll
+                        for (int p = 0; p < 3; p++)
+#pragma unroll
+                            for (int q = 0; q < 3; q++)
+                                c[k][p][q] =
+                                    (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
+                    filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+                }
             }
         }
         if (!nj.ok) break;

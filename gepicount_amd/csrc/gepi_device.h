@@ -64,7 +64,8 @@ struct CountArgs {
     int32_t ks0;              /* K steps of class 0 (fp4: class 1 starts at
                                * super-step f4_super(ks0))                         */
     int32_t cls0_high;        /* MFMA: class 0 encoded as 0x80 (else class 1 is)    */
-    int32_t pad;
+    int32_t f32;              /* fp4: class sizes with n0 * n1 <= 2^24, so that the
+                               * filter may work on the fp32 counts              */
     ScanState *st;
     Pend *pend;
     unsigned long long pcap;

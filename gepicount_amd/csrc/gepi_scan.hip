@@ -399,6 +399,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ca.n_ks = n_ks;
     ca.ks0 = path == 3 ? w0 / 4 : w0 / 2;
     ca.cls0_high = cls0_high > 0 ? 1 : 0;
+    ca.f32 = (int64_t)n0 * n1 <= (int64_t)1 << 24 ? 1 : 0;
     ca.st = p->d_state;
     ca.pend = p->d_pend;
     ca.pcap = pcap;
@@ -430,7 +431,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const bool entropy = par->ffun == GEPI_FFUNC_ENTROPY;
     /* fp4: the filter runs on the fp32 accumulators when every product of a
      * class-0 and a class-1 count is exact in float */
-    const bool f32 = (int64_t)n0 * n1 <= (int64_t)1 << 24;
+    const bool f32 = ca.f32 != 0;
     while (t0 < n_tiles) {
         const int64_t t1 = std::min<int64_t>(n_tiles, t0 + band);
         ca.tile_begin = t0;
