@@ -413,7 +413,11 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ea.llo = p->tabs.llo;
 
     const int64_t evals_per_tile = pairs_per_tile * 20;
-    int64_t max_band = std::max<int64_t>(64, (int64_t)(2.0e9 / (double)evals_per_tile));
+    /* Bands grow 4x per band up to 8e9 evaluations (195,000 fp4 tiles): each
+     * band costs a threshold selection and a host round trip (~0.2 ms), and
+     * at the final threshold a band's survivors stay far below the pending
+     * list (the next band is also sized from the measured survivor rate). */
+    int64_t max_band = std::max<int64_t>(64, (int64_t)(8.0e9 / (double)evals_per_tile));
     if (par->max_grid_area > 0)
         max_band = std::max<int64_t>(1, ((int64_t)par->max_grid_area << 20) / evals_per_tile);
     int64_t band = std::max<int64_t>(1, std::min<int64_t>(max_band, (2 * k) / evals_per_tile + 1));
