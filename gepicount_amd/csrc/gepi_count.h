@@ -387,12 +387,13 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
     auto step = [&](v4f (&acc)[2][3][3], int32_t u, int32_t np) {
         const uint4 *L = lds[u & 1];
         uint4 wA[2][3], wB[3];
+        /* B first: the first MFMA needs B and A[0][0] only */
+#pragma unroll
+        for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
-#pragma unroll
-        for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             if (k < np) {
@@ -600,12 +601,13 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         auto step = [&](v4f (&acc)[2][3][3], int32_t u, int32_t np) {
             const uint4 *L = lds[u & 1];
             uint4 wA[2][3], wB[3];
+            /* B first: the first MFMA needs B and A[0][0] only */
+#pragma unroll
+            for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
                 for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
-#pragma unroll
-            for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 if (k < np) {
