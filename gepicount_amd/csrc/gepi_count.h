@@ -329,7 +329,13 @@ template <int FFUN, bool F32>
 __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_ROWS * F4_PITCH];
     __shared__ PendStage stage;
+    __shared__ PairList pairs;
+    /* Fisher flags a variant pair in about 13% of lanes: those pairs are
+     * parked and finished one per thread after the tile (entropy's few are
+     * finished in the lane) */
+    constexpr bool PARK = FFUN == GEPI_FFUNC_FISHERPV;
     stage_init(stage);
+    if (threadIdx.x == 0) pairs.n = 0;
     int32_t ta, tb;
     tile_of_f4(tile, g.n_var_pad / TM, ta, tb);
 
@@ -463,7 +469,7 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 #pragma unroll
                         for (int q = 0; q < 3; q++)
                             c[k][p][q] = k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r];
-                filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+                filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, c, &pairs);
             } else {
                 int32_t c[2][3][3];
 #pragma unroll
@@ -473,9 +479,13 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 #pragma unroll
                         for (int q = 0; q < 3; q++)
                             c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
-                filter_pair<FFUN, false>(g, stage, fc, A, B, c);
+                filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, c, &pairs);
             }
         }
+    }
+    if (PARK) {
+        __syncthreads();
+        pair_drain<FFUN>(g, stage, fc, pairs);
     }
     stage_flush(g, stage);
 }

@@ -246,6 +246,18 @@ __device__ __forceinline__ int32_t fun_count(int f, int32_t n, int32_t a, int32_
     return f == 0 ? n : f == 1 ? b - n : f == 2 ? a - n : f == 3 ? v - a - b + n : a + b - 2 * n;
 }
 
+/* Fisher's exact check of one evaluation: log10 P(obs) <= thr (p >=
+ * P(observed table)) from the log-factorial table, with the admission's
+ * 1e-6 margin.                                                             */
+__device__ __forceinline__ bool fisher_admits(const CountArgs &g, const FilterConsts &fc, int32_t w,
+                                              int32_t x, int32_t y, int32_t z) {
+    const int32_t m = y + z;
+    const double *lf = g.lfact;
+    const double lp = ((lf[y] + lf[z]) + (lf[w + x] + lf[m - w - x])) -
+                      (((lf[w] + lf[y - w]) + (lf[x] + lf[z - x])) + lf[m]);
+    return lp * 0.4342944819032518 - 1e-6 <= fc.thr_s;
+}
+
 /* Appends the evaluations whose bit 5*rp+f is set in mask to the pending
  * list.  Runs only for lanes with a survivor.  The row pair's operands
  * advance through rotating registers: selecting among the counts with a
@@ -256,7 +268,7 @@ template <int FFUN>
 __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                           int32_t A, int32_t B, const int32_t (&c)[2][3][3],
                                           uint32_t mask) {
-    const int32_t y = c[0][2][2], z = c[1][2][2], m = y + z;
+    const int32_t y = c[0][2][2], z = c[1][2][2];
     int32_t n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
     int32_t n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
     int32_t a0 = c[0][0][2], a1 = c[1][0][2];
@@ -269,12 +281,8 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, con
             const int f = __builtin_ctz(mb);
             mb &= mb - 1;
             const int32_t w = fun_count(f, n0a, a0, b0, y), x = fun_count(f, n1a, a1, b1, z);
-            if (FFUN == GEPI_FFUNC_FISHERPV && fc.thr_s < 0.0) {
-                const double *lf = g.lfact;
-                const double lp = ((lf[y] + lf[z]) + (lf[w + x] + lf[m - w - x])) -
-                                  (((lf[w] + lf[y - w]) + (lf[x] + lf[z - x])) + lf[m]);
-                if (!(lp * 0.4342944819032518 - 1e-6 <= fc.thr_s)) continue;
-            }
+            if (FFUN == GEPI_FFUNC_FISHERPV && fc.thr_s < 0.0 && !fisher_admits(g, fc, w, x, y, z))
+                continue;
             push_pending(g, ps, w, x, y, z, base | (uint64_t)f);
         }
         n0a = n0b; n0b = n0c; n0c = n0d;
@@ -317,7 +325,9 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, con
  *
  * ROLLED: a runtime loop over the four row pairs keeps the code of callers
  * with many pairs per thread small; its operands advance through rotating
- * registers (an indexed array would be placed in scratch memory).          */
+ * registers (an indexed array would be placed in scratch memory).
+ * DEFER: a variant pair with a flagged row pair is parked in pl (its counts)
+ * for pair_drain instead of being finished in the lane.                    */
 template <typename DT>
 struct RowTest {
     float kyz, mf, bias;
@@ -368,9 +378,34 @@ __device__ __forceinline__ float sq_of(float q0, float q1) { return q0 + q1; }
 __device__ __forceinline__ float yz_of(int32_t y, int32_t z) { return (float)__mul24(y, z); }
 __device__ __forceinline__ float yz_of(float y, float z) { return y * z; }
 
-template <int FFUN, bool ROLLED, typename CT>
+struct PairList;
+template <int FFUN, bool ROLLED, bool DEFER = false, typename CT>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
-                                            int32_t A, int32_t B, const CT (&c)[2][3][3]) {
+                                            int32_t A, int32_t B, const CT (&c)[2][3][3],
+                                            PairList *pl = nullptr);
+
+/* Variant pairs with a flagged row pair, parked in LDS: the lanes that
+ * flag one are few and scattered over the waves, and finishing a pair (the
+ * function bits, Fisher's exact check, the pending list) in the lane costs
+ * each wave its full length for them.  After the tile the workgroup
+ * finishes the parked pairs one per thread (pair_drain); past PAIR_LDS the
+ * lane finishes at once.  Counts are below 2^16 (class sizes <= 46,340). */
+constexpr int PAIR_LDS = 512;
+struct PairRec {
+    uint16_t c[2][3][3];
+    int32_t A, B;
+    uint32_t pad;
+};
+struct PairList {
+    PairRec rec[PAIR_LDS];
+    uint32_t n;
+    uint32_t pad[3];
+};
+
+template <int FFUN, bool ROLLED, bool DEFER, typename CT>
+__device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
+                                            int32_t A, int32_t B, const CT (&c)[2][3][3],
+                                            PairList *pl) {
     const CT y = c[0][2][2], z = c[1][2][2];
     const CT m = y + z;
     const float mf = (float)m;
@@ -416,6 +451,21 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
                (rt.any(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) ? 8u : 0u);
     }
     if (A == B) rows &= ~2u; /* (2A, 2A+1) is not a row pair */
+    if (DEFER && rows) {
+        const uint32_t pos = atomicAdd(&pl->n, 1u);
+        if (pos < (uint32_t)PAIR_LDS) {
+            PairRec &r = pl->rec[pos];
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) r.c[k][p][q] = (uint16_t)c[k][p][q];
+            r.A = A;
+            r.B = B;
+            return;
+        }
+    }
     if (rows) {
         /* rare: the function bits of the flagged row pairs */
         uint32_t mask = 0;
@@ -435,6 +485,28 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
 #pragma unroll
                 for (int q = 0; q < 3; q++) ci[k][p][q] = (int32_t)c[k][p][q];
         push_mask<FFUN>(g, ps, fc, A, B, ci, mask);
+    }
+}
+
+/* Finishes the parked pairs, one per thread, with the int32 filter (as
+ * conservative as the test that parked them, so it keeps every evaluation
+ * that the exact top-k needs).  Every thread of the workgroup, after a
+ * barrier that follows the last park; the caller clears pl.n after a later
+ * barrier.                                                                 */
+template <int FFUN>
+__device__ __forceinline__ void pair_drain(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
+                                           const PairList &pl) {
+    const uint32_t n = pl.n < (uint32_t)PAIR_LDS ? pl.n : (uint32_t)PAIR_LDS;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const PairRec &r = pl.rec[i];
+        int32_t c[2][3][3];
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) c[k][p][q] = r.c[k][p][q];
+        filter_pair<FFUN, true>(g, ps, fc, r.A, r.B, c);
     }
 }
 

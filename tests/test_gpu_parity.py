@@ -126,6 +126,25 @@ def test_fp4_filter_count_types(n0, n1, ffun):
     assert_same(got, ref, "fp4-filter")
 
 
+def test_fisher_parked_pairs_overflow(monkeypatch):
+    """The fp4 single-scan kernel parks variant pairs that pass the
+    chi-square bound in LDS and finishes them (function bits, Fisher's exact
+    check) one per thread after the tile; with a loose threshold (large k,
+    two-tile bands) a tile flags more pairs than the list holds and the rest
+    are finished in the lane.  Together they must keep exactly the oracle's
+    top-k."""
+    print("case fisher parked pairs", flush=True)
+    bits, valid, m0, m1 = random_panel(57, 300, 400, 0.02)
+    k = 50000
+    monkeypatch.setenv("GEPI_BAND_TILES", "2")
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, 400, topmost=k,
+                               ffun=ge.GEPI_FFUNC_FISHERPV, count_path=3)
+    monkeypatch.delenv("GEPI_BAND_TILES")
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 400), k, ffun=1)
+    assert st["bands"] > 2
+    assert_same(got, ref, "fisher-parked")
+
+
 def test_kat_test_fixture_top5():
     d = ge.read_plink(os.path.join(DATA, "test"))
     got, _ = ge.process_gpu_1(d.bit_data, d.valid_data, d.mask_ctrl, d.mask_case, d.n_samples,
