@@ -228,10 +228,20 @@ GEPI_HD double fisher_exact(int32_t w, int32_t x, int32_t y, int32_t z) {
     double s_tail = 1.0;
     int32_t n_in = 1;
     XF cur = one;
+    /* Each walk may stop early without changing a bit of the result: past
+     * the mode (step ratio <= 1, and the ratio only falls further along a
+     * walk, so the computed terms never grow again) a term below 2^-55 is
+     * under half an ulp of s_all (>= 1) and of s_tail (>= 1), so it and every
+     * term after it leave both sums as they are, and each of them is in the
+     * tail (T <= 1 + 1e-7): only their count is needed.                    */
     for (int32_t j = a; j < hi; j++) {
         double num = (double)(c0 - j) * (double)(r0 - j);
         double den = (double)(j + 1) * (double)(c1 - r0 + j + 1);
         cur = xf_norm(cur.t * (num / den), cur.e);
+        if (cur.e <= -56 && num <= den) {
+            n_in += hi - j; /* terms j+1 .. hi */
+            break;
+        }
         s_all = xf_add(s_all, cur);
         if (xf_le_tol(cur)) {
             s_tail = s_tail + xf_to_double(cur);
@@ -243,6 +253,10 @@ GEPI_HD double fisher_exact(int32_t w, int32_t x, int32_t y, int32_t z) {
         double num = (double)(c0 - j + 1) * (double)(r0 - j + 1);
         double den = (double)j * (double)(c1 - r0 + j);
         cur = xf_norm(cur.t * (den / num), cur.e);
+        if (cur.e <= -56 && den <= num) {
+            n_in += j - lo; /* terms j-1 .. lo */
+            break;
+        }
         s_all = xf_add(s_all, cur);
         if (xf_le_tol(cur)) {
             s_tail = s_tail + xf_to_double(cur);
