@@ -30,7 +30,8 @@ EXPORTS = ("gepi_params_init", "gepi_process_gpu_1", "gepi_panel_create", "gepi_
            "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_panel_qa",
            "gepi_panel_filter", "gepi_plink_open",
            "gepi_plink_close", "gepi_plink_check_quality", "gepi_print_interactions",
-           "gepi_fisher_pval", "gepi_last_error", "gepi_version", "gepi_device_count",
+           "gepi_fisher_pval", "gepi_score_table", "gepi_last_error", "gepi_version",
+           "gepi_device_count",
            "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch",
            "gepi_server_create", "gepi_server_destroy", "gepi_server_add_panel_bed",
            "gepi_server_release_panel", "gepi_server_submit", "gepi_server_done",
@@ -156,6 +157,7 @@ def lib():
     l.gepi_last_error.restype = ctypes.c_char_p
     l.gepi_version.restype = ctypes.c_char_p
     l.gepi_device_count.argtypes = [ctypes.POINTER(ctypes.c_int32)]
+    l.gepi_score_table.argtypes = [ctypes.c_int32] * 5 + [ctypes.POINTER(ctypes.c_double)]
     l.gepi_engine_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp)]
     l.gepi_engine_destroy.argtypes = [vp]
     l.gepi_engine_scan_batch.argtypes = [vp, ctypes.POINTER(BatchItem), ctypes.c_int32,
@@ -205,6 +207,14 @@ def device_count():
     n = ctypes.c_int32(0)
     _check(lib().gepi_device_count(ctypes.byref(n)))
     return n.value
+
+
+def score_table(w, x, y, z, ffun=GEPI_FFUNC_ENTROPY):
+    """Exact objective of one table (what a result's obj holds for these
+    counts): -I(F;C) in bits, or log10 two-sided Fisher p.  Host only."""
+    s = ctypes.c_double(0.0)
+    _check(lib().gepi_score_table(int(w), int(x), int(y), int(z), int(ffun), ctypes.byref(s)))
+    return s.value
 
 
 def _result_buffer(topmost, n_rows):

@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -829,4 +830,35 @@ extern "C" int gepi_process_gpu_1(const uint64_t *bit_data, const uint64_t *vali
     rc = run_scan(p, mask_ctrl, mask_case, par, tops, n_tops, stats);
     panel_free(p);
     return rc;
+}
+
+/* One table scored on the host by the exact stage's own code (gepi_score.h,
+ * built here by the same compiler and flags as the device code). */
+extern "C" int gepi_score_table(int32_t w, int32_t x, int32_t y, int32_t z, int32_t ffun,
+                                double *score) {
+    if (!score) return gepi_fail(GEPI_ERR_ARG, "score is NULL");
+    if (w < 0 || x < 0 || w > y || x > z)
+        return gepi_fail(GEPI_ERR_ARG, "(w, x, y, z) = (%d, %d, %d, %d) is not a table", w, x, y, z);
+    if ((int64_t)y + z > GEPI_MAX_SAMPLES)
+        return gepi_fail(GEPI_ERR_LIMIT, "%lld samples exceed the limit of %d", (long long)y + z,
+                         GEPI_MAX_SAMPLES);
+    if (ffun == GEPI_FFUNC_FISHERPV) {
+        *score = gepi::fisher_exact(w, x, y, z);
+        return GEPI_OK;
+    }
+    if (ffun != GEPI_FFUNC_ENTROPY) return gepi_fail(GEPI_ERR_ARG, "unknown objective %d", ffun);
+    /* t log2 t tables, grown to the largest table scored so far */
+    static std::mutex mu;
+    static std::vector<double> lhi, llo, lf;
+    std::lock_guard<std::mutex> lock(mu);
+    const size_t n = (size_t)y + z + 1;
+    if (lhi.size() < n) {
+        const size_t m = std::max<size_t>(n, 2 * lhi.size());
+        lhi.assign(m, 0.0);
+        llo.assign(m, 0.0);
+        lf.assign(m, 0.0);
+        gepi_build_tables((int32_t)m, lhi.data(), llo.data(), lf.data());
+    }
+    *score = gepi::entropy_exact(w, x, y, z, lhi.data(), llo.data());
+    return GEPI_OK;
 }

@@ -277,6 +277,15 @@ int gepi_print_interactions(const gepi_plink *p, const gepi_result *tops, int64_
 /* fisher_pval (src/misc/fisher_p.c:52-76, declared inc/gEpiCount.h:137-139):
  * log10 two-sided Fisher p of the table (n11, n12; n21, n22), as float.      */
 float gepi_fisher_pval(int32_t n11, int32_t n12, int32_t n21, int32_t n22);
+/* The exact objective of one class x function table, as a result's obj holds
+ * it: w, x = function-true samples of class 0 and class 1, y, z = class
+ * sizes; ffun GEPI_FFUNC_ENTROPY gives -I(F;C) in bits (the per-evaluation
+ * entropy of src/kernels/entropy_inline.h:30-80), GEPI_FFUNC_FISHERPV log10 of
+ * the two-sided Fisher p (src/kernels/fisher_d.cu:249-285).  Computed on the
+ * host by the code of the scan's exact stage, so it equals the obj of any
+ * result with these counts bit for bit.  GEPI_ERR_ARG unless 0 <= w <= y and
+ * 0 <= x <= z; GEPI_ERR_LIMIT past GEPI_MAX_SAMPLES.                        */
+int gepi_score_table(int32_t w, int32_t x, int32_t y, int32_t z, int32_t ffun, double *score);
 
 /* ---- misc ---------------------------------------------------------------- */
 const char *gepi_last_error(void);

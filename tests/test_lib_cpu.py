@@ -57,6 +57,45 @@ def test_fisher_pval_matches_reference_binary_fixture():
         assert got == bits, (n11, n12, n21, n22)
 
 
+def _bits(v):
+    return struct.pack("<d", v)
+
+
+def test_score_table_equals_oracle():
+    """gepi_score_table runs the scan's exact stage (gepi_score.h, built with
+    the device code's compiler and flags) on the host.  Every table scores as
+    the oracle's restatement, bit for bit: all tables up to 12 + 12 samples,
+    then random ones up to 9,000 + 9,000 (near-proportional ones cross the
+    mode of the Fisher walks, whose early exits must not change a bit)."""
+    rng = np.random.default_rng(91)
+    tables = [(w, x, y, z) for y in range(13) for z in range(13)
+              for w in range(y + 1) for x in range(z + 1)]
+    for i in range(3000):
+        y, z = (int(v) for v in rng.integers(1, 9001, size=2))
+        if i % 3 == 0:
+            w, x = int(rng.integers(0, y + 1)), int(rng.integers(0, z + 1))
+        else:
+            f = rng.random()
+            w = int(f * y)
+            x = int(min(max((f + rng.uniform(-0.1, 0.1)) * z, 0), z))
+        tables.append((w, x, y, z))
+    for w, x, y, z in tables:
+        assert _bits(ge.score_table(w, x, y, z, ge.GEPI_FFUNC_ENTROPY)) == \
+            _bits(orc.entropy_exact(w, x, y, z)), (w, x, y, z)
+        assert _bits(ge.score_table(w, x, y, z, ge.GEPI_FFUNC_FISHERPV)) == \
+            _bits(orc.fisher_rank(w, x, y, z)), (w, x, y, z)
+
+
+def test_score_table_rejects_bad_tables():
+    for args in [(-1, 0, 3, 3), (4, 0, 3, 3), (0, 4, 3, 3), (0, 0, -1, 3)]:
+        with pytest.raises(ge.GepiError):
+            ge.score_table(*args)
+    with pytest.raises(ge.GepiError):
+        ge.score_table(0, 0, 30000, 30000)
+    with pytest.raises(ge.GepiError):
+        ge.score_table(1, 1, 3, 3, ffun=7)
+
+
 @pytest.mark.parametrize("name", ["test", "thinhap1"])
 def test_reader_equals_oracle_reader(name):
     root = os.path.join(DATA, name)
