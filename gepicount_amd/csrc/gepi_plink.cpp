@@ -322,11 +322,21 @@ extern "C" int gepi_plink_check_quality(const gepi_plink *p, float max_miss_snps
     const int64_t R = p->n_rows;
     const int32_t n = p->n_samples;
     int32_t bad_s = 0, bad_r = 0;
+    /* missing calls per sample in one pass over the rows: the set bits of
+     * each row's ~valid within the n samples (a column walk per sample
+     * reads the whole panel n times) */
+    std::vector<int64_t> miss_s((size_t)n, 0);
+    for (int64_t r = 0; r < R; r++) {
+        const uint64_t *v = p->valid_data + (size_t)r * W;
+        for (int32_t wd = 0; wd < W; wd++) {
+            const int32_t s0 = wd * 64;
+            uint64_t m = ~v[wd];
+            if (n - s0 < 64) m &= (1ull << (n - s0)) - 1;
+            for (; m; m &= m - 1) miss_s[(size_t)s0 + __builtin_ctzll(m)]++;
+        }
+    }
     for (int32_t s = 0; s < n; s++) {
-        int64_t valid = 0;
-        uint64_t bit = 1ull << (s & 63);
-        for (int64_t r = 0; r < R; r++)
-            if (p->valid_data[(size_t)r * W + (s >> 6)] & bit) valid++;
+        const int64_t valid = R - miss_s[(size_t)s];
         float miss = R ? (float)(R - valid) / (float)R : 0.0f;
         if (miss > max_miss_samples) {
             fprintf(stderr, "Sample %d:\t%s(%d)\tnMiss:%d\t(%4g)\n", s, p->sample_name[s],

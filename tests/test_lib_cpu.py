@@ -151,6 +151,27 @@ def test_check_quality_counts(tmp_path):
     assert bad_s == 1 and bad_r == 2
 
 
+def test_check_quality_counts_ragged_samples(tmp_path):
+    """Missing-call counts per sample and per row against numpy, with a sample
+    count that leaves a partial last word (150) and thresholds around the
+    observed fractions, so that the float comparisons decide."""
+    spec = dict(n_var=90, n_samples=150, n_case=70, maf=("uniform", 0.05, 0.5), miss=0.06,
+                planted=0, seed=71)
+    p = synth.make_panel(spec=spec)
+    root = str(tmp_path / "qa_ragged")
+    synth.write_plink(root, p)
+    d = ge.read_plink(root)
+    R, n = d.n_rows, d.n_samples
+    bits = np.unpackbits(d.valid_data.view(np.uint8).reshape(R, -1), axis=1,
+                         bitorder="little")[:, :n].astype(np.int64)
+    f_s = (R - bits.sum(axis=0)).astype(np.float32) / np.float32(R)
+    f_r = (n - bits.sum(axis=1)).astype(np.float32) / np.float32(n)
+    for thr in (0.03, 0.05, 0.06, 0.07, 0.09):
+        bad_s, bad_r = d.check_quality(thr, thr)
+        assert bad_s == int((f_s > np.float32(thr)).sum()), thr
+        assert bad_r == int((f_r > np.float32(thr)).sum()), thr
+
+
 def test_report_format_matches_oracle(tmp_path):
     """printInteractions output of the library == the oracle's print_misc.h restatement."""
     root = os.path.join(DATA, "thinhap1")
