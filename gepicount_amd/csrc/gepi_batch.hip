@@ -385,7 +385,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             d.g.n_var_pad = (int32_t)it->n_var_pad;
             d.g.n_ks = it->n_k4;
             d.g.ks0 = it->ks0;
-            d.g.f32 = (int64_t)it->n0 * it->n1 <= (int64_t)1 << 24 ? 1 : 0;
+            d.g.f32 = f32_counts(it->n0, it->n1) ? 1 : 0;
             d.g.st = it->d_state;
             d.g.pend = it->d_pend;
             d.g.pcap = it->pcap;

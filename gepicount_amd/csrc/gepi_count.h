@@ -324,7 +324,7 @@ __device__ __forceinline__ v8i frag_f4(const uint4 &w, uint32_t m) {
 /* One 64 x 32-variant workgroup tile of the fp4 path (tile numbering as in
  * tile_of_f4).  Called by every thread of a 256-thread workgroup.  F32: the
  * filter works on the fp32 accumulators directly (the scan's class sizes
- * have n0 n1 <= 2^24), else on their int32 conversions.                    */
+ * pass f32_counts), else on their int32 conversions.                       */
 template <int FFUN, bool F32>
 __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_ROWS * F4_PITCH];

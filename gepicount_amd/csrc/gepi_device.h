@@ -64,8 +64,8 @@ struct CountArgs {
     int32_t ks0;              /* K steps of class 0 (fp4: class 1 starts at
                                * super-step f4_super(ks0))                         */
     int32_t cls0_high;        /* MFMA: class 0 encoded as 0x80 (else class 1 is)    */
-    int32_t f32;              /* fp4: class sizes with n0 * n1 <= 2^24, so that the
-                               * filter may work on the fp32 counts              */
+    int32_t f32;              /* fp4: class sizes for which the filter may work on
+                               * the fp32 counts (f32_counts)                    */
     ScanState *st;
     Pend *pend;
     unsigned long long pcap;
@@ -331,6 +331,7 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, con
 template <typename DT>
 struct RowTest {
     float kyz, mf, bias;
+    float kyzm; /* float counts: kyz * mf, exact (see filter_pair) */
     /* s of the five functions of one row pair.  Every D below is one
      * function's w z - x y up to sign, so |D| <= y z: exact for int32 and,
      * when y z <= 2^24, for float.                                         */
@@ -342,7 +343,13 @@ struct RowTest {
         const float X[5] = {Sn, Sb - Sn, Sa - Sn, (mf - Sab) + Sn, Sab - 2.0f * Sn};
 #pragma unroll
         for (int f = 0; f < 5; f++) {
-            const float r = kyz * (X[f] * (mf - X[f]));
+            float r;
+            if constexpr (sizeof(DT) == sizeof(float) && DT(0.5f) != DT(0)) {
+                /* kyz (m - X) exactly in one fma, then one rounding */
+                r = X[f] * __builtin_fmaf(-kyz, X[f], kyzm);
+            } else {
+                r = kyz * (X[f] * (mf - X[f]));
+            }
             const float fd = (float)D[f];
             s[f] = __builtin_fmaf(fd, fd, -r);
         }
@@ -362,9 +369,16 @@ struct RowTest {
     }
 };
 
+/* Class sizes for which the fp4 filter works on its fp32 accumulators:
+ * every product of a class-0 and a class-1 count exact in float (n0 n1 <=
+ * 2^24) and m <= 16384 (the RowTest's exact kyz (m - X)).                  */
+__host__ __device__ inline bool f32_counts(int64_t n0, int64_t n1) {
+    return n0 * n1 <= ((int64_t)1 << 24) && n0 + n1 <= 16384;
+}
+
 /* Count arithmetic of the filter.  int32 counts (< 2^16): D = q0 z - q1 y
  * with 24-bit products.  float counts (the fp4 path's accumulators, exact
- * integers): when n0 n1 <= 2^24 every product q0 z, q1 y is at most n0 n1
+ * integers, f32_counts): every product q0 z, q1 y is at most n0 n1 <= 2^24
  * (a count of class c is at most n_c) and so exact, and fma gives D exactly
  * with no conversion.                                                     */
 __device__ __forceinline__ int32_t dq_of(int32_t q0, int32_t q1, int32_t y, int32_t z) {
@@ -421,6 +435,14 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
     RowTest<CT> rt;
     rt.kyz = K * yz_of(y, z) * (1.0f - 1.0f / 65536.0f);
     rt.mf = mf;
+    rt.kyzm = 0.0f;
+    if constexpr (sizeof(CT) == sizeof(float) && CT(0.5f) != CT(0)) {
+        /* float counts come with m <= 16384 (14 bits): kyz cut down to 10
+         * significant bits makes kyz m and kyz X exact, so that fma(-kyz, X,
+         * kyz m) is kyz (m - X) exactly; cutting only lowers the bound */
+        rt.kyz = __uint_as_float(__float_as_uint(rt.kyz) & 0xFFFFC000u);
+        rt.kyzm = rt.kyz * mf;
+    }
     rt.bias = K == 0.0f ? -1.0f : 0.0f;
     auto dq = [&](CT q0, CT q1) { return dq_of(q0, q1, y, z); };
     auto sq = [&](CT q0, CT q1) { return sq_of(q0, q1); };

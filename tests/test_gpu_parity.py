@@ -106,13 +106,14 @@ def test_fisher_objective(seed, n_var, n, miss, k, path):
 
 
 @pytest.mark.parametrize("ffun", [0, 1])
-@pytest.mark.parametrize("n0,n1", [(4096, 4096), (4200, 4200)])
+@pytest.mark.parametrize("n0,n1", [(4096, 4096), (4200, 4200), (1100, 15000), (1000, 16000)])
 def test_fp4_filter_count_types(n0, n1, ffun):
     """The fp4 path filters on its fp32 accumulators when every product of a
     class-0 and a class-1 count is exact in float (n0 n1 <= 2^24; 4096 x 4096
-    is the largest balanced case, where products reach 2^24) and on int32
-    conversions above that (4200 x 4200).  Both must admit every evaluation
-    the exact top-k needs."""
+    is the largest balanced case, where products reach 2^24) and the samples
+    number at most 16,384 (1100 + 15000 is near both limits), and on int32
+    conversions otherwise (4200 x 4200; 1000 + 16000 samples).  Both must
+    admit every evaluation the exact top-k needs."""
     print("case fp4 filter n0=%d n1=%d ffun=%d" % (n0, n1, ffun), flush=True)
     rng = np.random.default_rng(83)
     n = n0 + n1
