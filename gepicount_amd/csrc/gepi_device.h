@@ -304,10 +304,12 @@ __device__ __forceinline__ void push_mask(const CountArgs &g, PendStage &ps, con
  * I <= ln(1 + chi2/m), so I >= T requires D^2 >= expm1(T) * y * z * q.
  * Entropy: -I_bits <= thr needs T = -thr * ln 2 (kf = 2^-thr - 1, made on
  * the host side of the threshold update, rounded down).
- * Fisher: p >= P(obs) = C(y,w) C(z,x) / C(m,xx), and e^{nH(k/n)} / sqrt(2n)
- * <= C(n,k) <= e^{nH(k/n)} (H in nats) give P(obs) >= e^{-mI} / (2 sqrt(yz)),
- * so log10 P(obs) - 1e-6 <= thr needs
- *     T = (-(thr + 1e-6) ln 10 - ln(4yz) / 2) / m   (y, z clamped to >= 1);
+ * Fisher: p >= P(obs) = C(y,w) C(z,x) / C(m,xx); e^{nH(k/n)} / sqrt(2n) <=
+ * C(n,k) for the numerator and C(m,xx) <= e^{mH} sqrt(m / (2 pi xx (m-xx)))
+ * (1 <= xx <= m-1; H in nats) give P(obs) >= e^{-mI} sqrt(2 pi (m-1) / m) /
+ * (2 sqrt(yz)), so log10 P(obs) - 1e-6 <= thr needs
+ *     T = (-(thr + 1e-6) ln 10 - ln(4yz) / 2 + 0.9189 - 1/m) / m
+ * (y, z clamped to >= 1; ln(2 pi (m-1)/m) / 2 >= 0.9189 - 1/m);
  * its survivors are then checked exactly against the log-factorial table.
  * The test is s = fd^2 - RN(K*y*z*(1 - 2^-16) * q) > bias with fd = float(D)
  * squared exactly inside one fma.  fd is D exactly when |D| <= 2^24 (always
@@ -428,7 +430,11 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
         K = 0.0f;
         if (fc.thr_s < 0.0) {
             const float yz1 = (float)(y > 1 ? y : 1) * (float)(z > 1 ? z : 1);
-            const float T = (fc.a_thr - 0.5f * __logf(4.0f * yz1)) / (m > 1 ? mf : 1.0f);
+            /* + ln(2 pi (m-1)/m) / 2 >= 0.9189 - 1/m: C(m, xx) <= e^{m H}
+             * sqrt(m / (2 pi xx (m - xx))) for 1 <= xx <= m-1 (the other
+             * tables have P(obs) = 1 and D = 0) */
+            const float mc = m > 1 ? mf : 1.0f;
+            const float T = (fc.a_thr - 0.5f * __logf(4.0f * yz1) + (0.9189f - 1.0f / mc)) / mc;
             if (T > 0.0f) K = expm1f(T) * (1.0f - 1e-5f);
         }
     }

@@ -7,7 +7,9 @@ margins involved) over exhaustive small tables and random large ones:
 
   I_nats <= ln(1 + chi2/m)                                  (Jensen on KL)
   e^{n H(k/n)} / sqrt(2n) <= C(n, k) <= e^{n H(k/n)}        (H in nats)
-  ln P(obs) >= -m I_nats - ln(4 y z) / 2                    (Fisher bound)
+  C(n, k) <= e^{n H(k/n)} sqrt(n / (2 pi k (n-k)))          (1 <= k <= n-1)
+  ln P(obs) >= -m I_nats - ln(4 y z) / 2 + g(m)             (Fisher bound)
+  g(m) = ln(2 pi (m-1) / m) / 2 >= 0.9189 - 1/m             (1 <= xx <= m-1)
 
 with P(obs) = C(y,w) C(z,x) / C(m,xx) the hypergeometric probability of the
 observed 2x2 table (w of y controls and x of z cases carry the function).
@@ -68,6 +70,10 @@ def test_binomial_entropy_bounds_exhaustive():
             nh = n * H(int(k) / n)
             assert lc <= nh + 1e-9 * max(1.0, nh), (n, k)
             assert lc >= nh - 0.5 * math.log(2 * n) - 1e-9 * max(1.0, nh), (n, k)
+            if 0 < k < n:
+                k = int(k)
+                ub = nh + 0.5 * math.log(n / (2 * math.pi * k * (n - k)))
+                assert lc <= ub + 1e-9 * max(1.0, nh), (n, k)
 
 
 def test_jensen_chi2_bound():
@@ -89,23 +95,31 @@ def test_fisher_pobs_lower_bound():
         lp = lchoose(y, w) + lchoose(z, x) - lchoose(m, xx)
         bound = -m * mi_nats(w, x, y, z) - 0.5 * math.log(4 * max(y, 1) * max(z, 1))
         assert lp >= bound - 1e-9 * max(1.0, abs(lp)), (w, x, y, z, lp, bound)
+        if 0 < xx < m:
+            g = 0.5 * math.log(2 * math.pi * (m - 1) / m)
+            assert g >= 0.9189 - 1.0 / m
+            assert lp >= bound + g - 1e-9 * max(1.0, abs(lp)), (w, x, y, z, lp, bound + g)
 
 
 def test_fisher_chain_rejects_only_unreachable_tables():
     """The chain the kernel applies: for a threshold thr on log10 p, a table
-    with D^2 < expm1(T) y z q and T = (-(thr + 1e-6) ln 10 - ln(4yz)/2) / m
-    has log10 P(obs) - 1e-6 > thr, so its p-value (>= P(obs)) misses thr."""
+    with D^2 < expm1(T) y z q and T = (-(thr + 1e-6) ln 10 - ln(4yz)/2 +
+    0.9189 - 1/m) / m has log10 P(obs) - 1e-6 > thr, so its p-value (>= P(obs))
+    misses thr.  (Tables with xx = 0 or m have P(obs) = 1 and q = 0.)"""
     rng = np.random.default_rng(13)
     rejected = 0
     for thr in (-2.0, -5.0, -12.0):
         for w, x, y, z in tables(rng, 20000, 3000):
             m = y + z
             xx = w + x
-            T = (-(thr + 1e-6) * math.log(10) - 0.5 * math.log(4 * max(y, 1) * max(z, 1))) / m
+            T = (-(thr + 1e-6) * math.log(10) - 0.5 * math.log(4 * max(y, 1) * max(z, 1))
+                 + 0.9189 - 1.0 / m) / m
             if T <= 0:
                 continue
             d = w * z - x * y
             q = xx * (m - xx)
+            if q == 0:
+                continue
             if d * d < math.expm1(T) * y * z * q:
                 rejected += 1
                 lp10 = (lchoose(y, w) + lchoose(z, x) - lchoose(m, xx)) / math.log(10)
