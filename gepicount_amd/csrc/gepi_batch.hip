@@ -386,6 +386,8 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             d.g.n_ks = it->n_k4;
             d.g.ks0 = it->ks0;
             d.g.f32 = f32_counts(it->n0, it->n1) ? 1 : 0;
+            d.g.n0 = it->n0;
+            d.g.n1 = it->n1;
             d.g.st = it->d_state;
             d.g.pend = it->d_pend;
             d.g.pcap = it->pcap;

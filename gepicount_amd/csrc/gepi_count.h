@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(POPC_THREADS) k_count_popc(CountArgs g) {
         for (int kk = split; kk < end; kk += 4) popc_step<1>(sA, sB, ty, tx, kk, acc);
     }
 
-    const FilterConsts fc = filter_consts(g.st);
+    const FilterConsts fc = filter_consts(g.st, g.n0, g.n1);
 #pragma unroll
     for (int i = 0; i < 2; i++) {
 #pragma unroll
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(MF_THREADS, 2) k_count_mfma(CountArgs g) {
 
     /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
      * variant 32wa + 16i + 4lg + r against B variant 32wb + 16j + l16.     */
-    const FilterConsts fc = filter_consts(g.st);
+    const FilterConsts fc = filter_consts(g.st, g.n0, g.n1);
     const bool c0_high = g.cls0_high != 0; /* class 0 held in the high part */
 #pragma unroll
     for (int i = 0; i < 2; i++) {
@@ -452,7 +452,7 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 
     /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
      * variant 32wa + 16i + 4lg + r against B variant 16wb + l16.           */
-    const FilterConsts fc = filter_consts(g.st);
+    const FilterConsts fc = filter_consts(g.st, g.n0, g.n1);
 #pragma unroll
     for (int i = 0; i < 2; i++) {
 #pragma unroll
@@ -669,7 +669,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
 
         /* the filter's constants are read before the next tile's loads are
          * issued: waits on global loads are in issue order */
-        const FilterConsts fc = filter_consts(g.st);
+        const FilterConsts fc = filter_consts(g.st, g.n0, g.n1);
         /* next job (written before this tile's first barrier) and its first
          * super-step, in flight during the filter */
         const F4Job nj = uniform_job(s_job[(t + 1) & 1]);

@@ -66,6 +66,7 @@ struct CountArgs {
     int32_t cls0_high;        /* MFMA: class 0 encoded as 0x80 (else class 1 is)    */
     int32_t f32;              /* fp4: class sizes for which the filter may work on
                                * the fp32 counts (f32_counts)                    */
+    int32_t n0, n1;           /* class sizes (bounds of y and z: Fisher filter)   */
     ScanState *st;
     Pend *pend;
     unsigned long long pcap;
@@ -225,17 +226,20 @@ __device__ __forceinline__ void stage_flush(const CountArgs &g, PendStage &ps) {
 struct FilterConsts {
     double thr_s; /* admission score threshold                                 */
     float kf;     /* entropy: chi-square coefficient, 0 = admit everything      */
-    float a_thr;  /* Fisher: -(thr + 1e-6) * ln 10, less a float error margin   */
+    float a_s;    /* Fisher: -(thr + 1e-6) ln 10 - ln(4 n0 n1) / 2 + 0.9189,
+                   * less a float error margin                                 */
 };
 
-__device__ __forceinline__ FilterConsts filter_consts(const ScanState *st) {
+__device__ __forceinline__ FilterConsts filter_consts(const ScanState *st, int32_t n0, int32_t n1) {
     FilterConsts fc;
     fc.thr_s = st->thr_score;
     fc.kf = st->filt_k;
     const double a = -(fc.thr_s + 1e-6) * 2.302585092994046;
-    /* float rounding of a and of the per-pair log below stay under 1e-6
-     * relative; the margin keeps the derived bound conservative            */
-    fc.a_thr = fc.thr_s < 0.0 ? (float)(a - (fabs(a) + 10.0) * 1e-6) : 0.0f;
+    const double A = a - 0.5 * log(4.0 * (double)(n0 > 1 ? n0 : 1) * (double)(n1 > 1 ? n1 : 1)) +
+                     0.9189;
+    /* the float roundings of A and of the per-pair arithmetic stay far
+     * under this margin, which keeps the derived bound conservative        */
+    fc.a_s = fc.thr_s < 0.0 ? (float)(A - (fabs(A) + fabs(a) + 10.0) * 1e-6) : 0.0f;
     return fc;
 }
 
@@ -429,13 +433,14 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
     if (FFUN == GEPI_FFUNC_FISHERPV) {
         K = 0.0f;
         if (fc.thr_s < 0.0) {
-            const float yz1 = (float)(y > 1 ? y : 1) * (float)(z > 1 ? z : 1);
-            /* + ln(2 pi (m-1)/m) / 2 >= 0.9189 - 1/m: C(m, xx) <= e^{m H}
-             * sqrt(m / (2 pi xx (m - xx))) for 1 <= xx <= m-1 (the other
-             * tables have P(obs) = 1 and D = 0) */
-            const float mc = m > 1 ? mf : 1.0f;
-            const float T = (fc.a_thr - 0.5f * __logf(4.0f * yz1) + (0.9189f - 1.0f / mc)) / mc;
-            if (T > 0.0f) K = expm1f(T) * (1.0f - 1e-5f);
+            /* T = (a - ln(4yz)/2 + 0.9189 - 1/m) / m at ln(4yz) <= ln(4 n0 n1)
+             * (a_s; y <= n0, z <= n1), and K = T (1 + T/2) <= expm1(T): both
+             * only lower K; 0.9189 - 1/m <= ln(2 pi (m-1)/m) / 2 comes from
+             * C(m, xx) <= e^{m H} sqrt(m / (2 pi xx (m - xx))), 1 <= xx <= m-1
+             * (the other tables have P(obs) = 1 and D = 0) */
+            const float r = __builtin_amdgcn_rcpf(m > 1 ? mf : 1.0f);
+            const float T = (fc.a_s - r) * r;
+            if (T > 0.0f) K = T * (1.0f + 0.5f * T) * (1.0f - 2e-5f);
         }
     }
     RowTest<CT> rt;

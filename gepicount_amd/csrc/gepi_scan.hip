@@ -401,6 +401,8 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ca.ks0 = path == 3 ? w0 / 4 : w0 / 2;
     ca.cls0_high = cls0_high > 0 ? 1 : 0;
     ca.f32 = f32_counts(n0, n1) ? 1 : 0;
+    ca.n0 = n0;
+    ca.n1 = n1;
     ca.st = p->d_state;
     ca.pend = p->d_pend;
     ca.pcap = pcap;

@@ -10,6 +10,7 @@ margins involved) over exhaustive small tables and random large ones:
   C(n, k) <= e^{n H(k/n)} sqrt(n / (2 pi k (n-k)))          (1 <= k <= n-1)
   ln P(obs) >= -m I_nats - ln(4 y z) / 2 + g(m)             (Fisher bound)
   g(m) = ln(2 pi (m-1) / m) / 2 >= 0.9189 - 1/m             (1 <= xx <= m-1)
+  expm1(T) >= T (1 + T/2)                                   (T >= 0)
 
 with P(obs) = C(y,w) C(z,x) / C(m,xx) the hypergeometric probability of the
 observed 2x2 table (w of y controls and x of z cases carry the function).
@@ -102,17 +103,22 @@ def test_fisher_pobs_lower_bound():
 
 
 def test_fisher_chain_rejects_only_unreachable_tables():
-    """The chain the kernel applies: for a threshold thr on log10 p, a table
-    with D^2 < expm1(T) y z q and T = (-(thr + 1e-6) ln 10 - ln(4yz)/2 +
-    0.9189 - 1/m) / m has log10 P(obs) - 1e-6 > thr, so its p-value (>= P(obs))
-    misses thr.  (Tables with xx = 0 or m have P(obs) = 1 and q = 0.)"""
+    """The chain the kernel applies: for a threshold thr on log10 p and class
+    sizes n0 >= y, n1 >= z, a table with D^2 < T (1 + T/2) y z q and
+    T = (-(thr + 1e-6) ln 10 - ln(4 n0 n1)/2 + 0.9189 - 1/m) / m has
+    log10 P(obs) - 1e-6 > thr, so its p-value (>= P(obs)) misses thr.
+    (Tables with xx = 0 or m have P(obs) = 1 and q = 0.)"""
     rng = np.random.default_rng(13)
     rejected = 0
+    for t in rng.random(1000) * 3:
+        assert math.expm1(t) >= t * (1 + t / 2)
     for thr in (-2.0, -5.0, -12.0):
         for w, x, y, z in tables(rng, 20000, 3000):
             m = y + z
             xx = w + x
-            T = (-(thr + 1e-6) * math.log(10) - 0.5 * math.log(4 * max(y, 1) * max(z, 1))
+            n0 = y + int(rng.integers(0, 3)) * int(rng.integers(0, 50))
+            n1 = z + int(rng.integers(0, 3)) * int(rng.integers(0, 50))
+            T = (-(thr + 1e-6) * math.log(10) - 0.5 * math.log(4 * max(n0, 1) * max(n1, 1))
                  + 0.9189 - 1.0 / m) / m
             if T <= 0:
                 continue
@@ -120,7 +126,7 @@ def test_fisher_chain_rejects_only_unreachable_tables():
             q = xx * (m - xx)
             if q == 0:
                 continue
-            if d * d < math.expm1(T) * y * z * q:
+            if d * d < T * (1 + T / 2) * y * z * q:
                 rejected += 1
                 lp10 = (lchoose(y, w) + lchoose(z, x) - lchoose(m, xx)) / math.log(10)
                 assert lp10 - 1e-6 > thr, (thr, w, x, y, z, lp10)
