@@ -352,15 +352,19 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
     /* Staging.  A super-step of a tile is 288 plane rows of 64 bytes (4
      * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
      * contiguous in the layout.  uint4 i < 768 of the A rows and i < 384 of
-     * the B rows; thread t takes A i = t + 256j (j < 3) and B i = t, t + 256
-     * (t < 128), i.e. row i/4, column t%4.                                  */
+     * the B rows; thread t takes A i = si + 256j (j < 3) and B i = si,
+     * si + 256 (t < 128), row i/4, column i%4, where si = 4 (16w + t%16) +
+     * (t/16)%4 (w = t/64) permutes each wave's 64 uint4 so that a 16-lane
+     * group writes one column of 16 consecutive rows: rows are 80 bytes
+     * apart, so the group's 16-byte stores hit 16 distinct bank groups.    */
     const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
     const int64_t ustride = (int64_t)g.n_var_pad * 3 * 4; /* uint4 per super-step */
-    const uint4 *gA = base + (int64_t)ta * (F4_RA * 4) + tid;
-    const uint4 *gB = base + (int64_t)tb * (TB4 * 3 * 4) + tid;
+    const int si = 4 * (((tid >> 6) << 4) + (tid & 15)) + ((tid >> 4) & 3);
+    const uint4 *gA = base + (int64_t)ta * (F4_RA * 4) + si;
+    const uint4 *gB = base + (int64_t)tb * (TB4 * 3 * 4) + si;
     const bool hasB1 = tid < 128;
-    const int dA = (tid >> 2) * F4_PITCH + (tid & 3);
-    const int dB = (F4_RA + (tid >> 2)) * F4_PITCH + (tid & 3);
+    const int dA = (si >> 2) * F4_PITCH + (si & 3);
+    const int dB = (F4_RA + (si >> 2)) * F4_PITCH + (si & 3);
     constexpr int D64 = 64 * F4_PITCH; /* 64 rows further down */
     uint4 pa0, pa1, pa2, pb0, pb1 = make_uint4(0u, 0u, 0u, 0u);
     auto fetch = [&](int32_t u) {
@@ -540,11 +544,15 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     /* Staging.  A super-step of a tile is 288 plane rows of 64 bytes (4
      * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
      * contiguous in the layout.  uint4 i < 768 of the A rows and i < 384 of
-     * the B rows; thread t takes A i = t + 256j (j < 3) and B i = t, t + 256
-     * (t < 128), i.e. row i/4, column t%4.                                  */
+     * the B rows; thread t takes A i = si + 256j (j < 3) and B i = si,
+     * si + 256 (t < 128), row i/4, column i%4, where si = 4 (16w + t%16) +
+     * (t/16)%4 (w = t/64) permutes each wave's 64 uint4 so that a 16-lane
+     * group writes one column of 16 consecutive rows: rows are 80 bytes
+     * apart, so the group's 16-byte stores hit 16 distinct bank groups.    */
+    const int si = 4 * (((tid >> 6) << 4) + (tid & 15)) + ((tid >> 4) & 3);
     const bool hasB1 = tid < 128;
-    const int dA = (tid >> 2) * F4_PITCH + (tid & 3);
-    const int dB = (F4_RA + (tid >> 2)) * F4_PITCH + (tid & 3);
+    const int dA = (si >> 2) * F4_PITCH + (si & 3);
+    const int dB = (F4_RA + (si >> 2)) * F4_PITCH + (si & 3);
     constexpr int D64 = 64 * F4_PITCH; /* 64 rows further down */
     /* lane's fragment rows: A (i, p) is row (32wa + 16i + l16)*3 + p, B q is
      * row 192 + (16wb + l16)*3 + q; the lane takes 16-byte column lg.      */
@@ -562,8 +570,8 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         tile_of_f4(tile, g.n_var_pad / TM, o.ta, o.tb);
         const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
         o.ustride = (int64_t)g.n_var_pad * 3 * 4; /* uint4 per super-step */
-        o.gA = base + (int64_t)o.ta * (F4_RA * 4) + tid;
-        o.gB = base + (int64_t)o.tb * (TB4 * 3 * 4) + tid;
+        o.gA = base + (int64_t)o.ta * (F4_RA * 4) + si;
+        o.gB = base + (int64_t)o.tb * (TB4 * 3 * 4) + si;
         o.ks1 = g.n_ks - g.ks0;
         o.u1 = f4_super(g.ks0);
         o.n_u = o.u1 + f4_super(o.ks1);
