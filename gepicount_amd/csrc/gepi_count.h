@@ -302,14 +302,22 @@ constexpr int F4_THREADS = 256; /* 4 waves, 2 x 2 of 32 x 16 variants           
 constexpr int TB4 = 32;         /* B variants per workgroup tile (A: TM = 64)     */
 constexpr int F4_RA = TM * 3;   /* A plane rows of a tile                         */
 constexpr int F4_ROWS = F4_RA + TB4 * 3; /* plane rows of a tile: A, then B      */
-constexpr int F4_PITCH = 5;     /* uint4 per LDS row: 4 of data, 1 of padding that
-                                   spreads the 16-byte fragment reads of 8
-                                   consecutive variants over distinct banks     */
+constexpr int F4_UNITS = F4_ROWS * 4; /* uint4 of a staged super-step             */
 constexpr int F4_FMT = 4;       /* operand format code of e2m1                    */
 
 /* fp4 layout geometry: a super-step is 3 K steps of 128 samples; each class
  * starts on a super-step, class 0 at 0 and class 1 at f4_super(ks0).       */
 __host__ __device__ inline int32_t f4_super(int32_t ks) { return (ks + 2) / 3; }
+
+/* LDS layout of a staged super-step: the 288 rows of 4 uint4 unpadded, uint4
+ * c of row r at unit 4r + (c ^ ((r >> 2) & 3)).  A fragment read (16 lanes:
+ * rows 3 apart, one column) and the staging stores (consecutive units) hit
+ * distinct bank groups whether the LDS serves 8, 16 or 32 lanes of a
+ * 16-byte access at a time.  Thread t of the workgroup stores units t + 256j
+ * of the A rows and of the B rows; the uint4 it loads for them is row
+ * t/4 + 64j, column (t%4) ^ ((t/16)%4), i.e. offset f4_src(t) + 256j.     */
+__device__ __forceinline__ int f4_unit(int r, int c) { return 4 * r + (c ^ ((r >> 2) & 3)); }
+__device__ __forceinline__ int f4_src(int t) { return 4 * (t >> 2) + ((t & 3) ^ ((t >> 4) & 3)); }
 
 /* One lane's fp4 fragment of K step p of a super-step: the lane's four
  * layout dwords masked to nibble bit p.  A set bit is e2m1 0.5 * 2^p; with
@@ -327,7 +335,7 @@ __device__ __forceinline__ v8i frag_f4(const uint4 &w, uint32_t m) {
  * pass f32_counts), else on their int32 conversions.                       */
 template <int FFUN, bool F32>
 __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) {
-    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_ROWS * F4_PITCH];
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_UNITS];
     __shared__ PendStage stage;
     __shared__ PairList pairs;
     /* Fisher flags a variant pair in about 13% of lanes: those pairs are
@@ -351,21 +359,16 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 
     /* Staging.  A super-step of a tile is 288 plane rows of 64 bytes (4
      * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
-     * contiguous in the layout.  uint4 i < 768 of the A rows and i < 384 of
-     * the B rows; thread t takes A i = si + 256j (j < 3) and B i = si,
-     * si + 256 (t < 128), row i/4, column i%4, where si = 4 (16w + t%16) +
-     * (t/16)%4 (w = t/64) permutes each wave's 64 uint4 so that a 16-lane
-     * group writes one column of 16 consecutive rows: rows are 80 bytes
-     * apart, so the group's 16-byte stores hit 16 distinct bank groups.    */
+     * contiguous in the layout; thread t stores LDS units t + 256j of the A
+     * rows (j < 3) and of the B rows (t + 256j < 384), see f4_unit.        */
     const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
     const int64_t ustride = (int64_t)g.n_var_pad * 3 * 4; /* uint4 per super-step */
-    const int si = 4 * (((tid >> 6) << 4) + (tid & 15)) + ((tid >> 4) & 3);
+    const int si = f4_src(tid);
     const uint4 *gA = base + (int64_t)ta * (F4_RA * 4) + si;
     const uint4 *gB = base + (int64_t)tb * (TB4 * 3 * 4) + si;
     const bool hasB1 = tid < 128;
-    const int dA = (si >> 2) * F4_PITCH + (si & 3);
-    const int dB = (F4_RA + (si >> 2)) * F4_PITCH + (si & 3);
-    constexpr int D64 = 64 * F4_PITCH; /* 64 rows further down */
+    const int dA = tid, dB = F4_RA * 4 + tid;
+    constexpr int D64 = 256; /* 64 rows further down */
     uint4 pa0, pa1, pa2, pb0, pb1 = make_uint4(0u, 0u, 0u, 0u);
     auto fetch = [&](int32_t u) {
         const int64_t o = (int64_t)u * ustride;
@@ -389,8 +392,12 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 
     /* lane's fragment rows: A (i, p) is row (32wa + 16i + l16)*3 + p, B q is
      * row 192 + (16wb + l16)*3 + q; the lane takes 16-byte column lg.      */
-    const int fA = (32 * wa + l16) * 3 * F4_PITCH + lg;
-    const int fB = (F4_RA + (16 * wb + l16) * 3) * F4_PITCH + lg;
+    int oA[3], oB[3]; /* A (i, p) at oA[p] + 192 i: 48 rows keep the swizzle */
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        oA[p] = f4_unit((32 * wa + l16) * 3 + p, lg);
+        oB[p] = f4_unit(F4_RA + (16 * wb + l16) * 3 + p, lg);
+    }
     /* super-step u into one class's accumulators, its first np K steps
      * (the rest of a class's last super-step is padding); the staging
      * pipeline runs on across the two calls.                               */
@@ -399,11 +406,11 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
         uint4 wA[2][3], wB[3];
         /* B first: the first MFMA needs B and A[0][0] only */
 #pragma unroll
-        for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
+        for (int q = 0; q < 3; q++) wB[q] = L[oB[q]];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
+            for (int p = 0; p < 3; p++) wA[i][p] = L[oA[p] + 192 * i];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             if (k < np) {
@@ -532,7 +539,7 @@ __device__ __forceinline__ F4Job uniform_job(const F4Job &j) {
 
 template <int FFUN, class Src>
 __device__ __forceinline__ void count_f4_run(const Src &src) {
-    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_ROWS * F4_PITCH];
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_UNITS];
     __shared__ PendStage stage;
     __shared__ F4Job s_job[2];
 
@@ -543,21 +550,20 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
 
     /* Staging.  A super-step of a tile is 288 plane rows of 64 bytes (4
      * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
-     * contiguous in the layout.  uint4 i < 768 of the A rows and i < 384 of
-     * the B rows; thread t takes A i = si + 256j (j < 3) and B i = si,
-     * si + 256 (t < 128), row i/4, column i%4, where si = 4 (16w + t%16) +
-     * (t/16)%4 (w = t/64) permutes each wave's 64 uint4 so that a 16-lane
-     * group writes one column of 16 consecutive rows: rows are 80 bytes
-     * apart, so the group's 16-byte stores hit 16 distinct bank groups.    */
-    const int si = 4 * (((tid >> 6) << 4) + (tid & 15)) + ((tid >> 4) & 3);
+     * contiguous in the layout; thread t stores LDS units t + 256j of the A
+     * rows (j < 3) and of the B rows (t + 256j < 384), see f4_unit.        */
+    const int si = f4_src(tid);
     const bool hasB1 = tid < 128;
-    const int dA = (si >> 2) * F4_PITCH + (si & 3);
-    const int dB = (F4_RA + (si >> 2)) * F4_PITCH + (si & 3);
-    constexpr int D64 = 64 * F4_PITCH; /* 64 rows further down */
+    const int dA = tid, dB = F4_RA * 4 + tid;
+    constexpr int D64 = 256; /* 64 rows further down */
     /* lane's fragment rows: A (i, p) is row (32wa + 16i + l16)*3 + p, B q is
      * row 192 + (16wb + l16)*3 + q; the lane takes 16-byte column lg.      */
-    const int fA = (32 * wa + l16) * 3 * F4_PITCH + lg;
-    const int fB = (F4_RA + (16 * wb + l16) * 3) * F4_PITCH + lg;
+    int oA[3], oB[3]; /* A (i, p) at oA[p] + 192 i: 48 rows keep the swizzle */
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        oA[p] = f4_unit((32 * wa + l16) * 3 + p, lg);
+        oB[p] = f4_unit(F4_RA + (16 * wb + l16) * 3 + p, lg);
+    }
 
     /* geometry of one tile */
     struct Geo {
@@ -621,11 +627,11 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             uint4 wA[2][3], wB[3];
             /* B first: the first MFMA needs B and A[0][0] only */
 #pragma unroll
-            for (int q = 0; q < 3; q++) wB[q] = L[fB + q * F4_PITCH];
+            for (int q = 0; q < 3; q++) wB[q] = L[oB[q]];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int p = 0; p < 3; p++) wA[i][p] = L[fA + (48 * i + p) * F4_PITCH];
+                for (int p = 0; p < 3; p++) wA[i][p] = L[oA[p] + 192 * i];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 if (k < np) {
