@@ -259,6 +259,32 @@ def test_candidate_overflow_recovery(monkeypatch, path):
     assert_same(got, ref, "candidate-overflow")
 
 
+@pytest.mark.parametrize("cand_cap", [None, "1"])
+def test_fp4_tile_panels_and_band_edges(monkeypatch, cand_cap):
+    """fp4 tiles are numbered in panels of 8 A rows (a shared rectangle
+    column by column, then the rows' remaining tiles); 620 variants give 10 A
+    rows, a full panel and a partial one of 2.  Bands of 7 tiles straddle
+    panel and row boundaries; with the candidate buffer forced to overflow,
+    the records of a band are dropped through the inverse of that numbering
+    and the band is redone.  Lost or duplicated tiles would break the exact
+    match."""
+    print("case fp4 panels cand_cap=%s" % cand_cap, flush=True)
+    bits, valid, m0, m1 = random_panel(61, 620, 200, 0.02)
+    k = 6000
+    monkeypatch.setenv("GEPI_BAND_TILES", "7")
+    if cand_cap:
+        monkeypatch.setenv("GEPI_CAND_CAP", cand_cap)
+        monkeypatch.setenv("GEPI_PEND_CAP", "4000000")
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, 200, topmost=k, count_path=3)
+    for v in ("GEPI_BAND_TILES", "GEPI_CAND_CAP", "GEPI_PEND_CAP"):
+        monkeypatch.delenv(v, raising=False)
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 200), k)
+    assert st["bands"] > 10
+    if cand_cap:
+        assert st["cand_overflows"] > 0
+    assert_same(got, ref, "fp4-panels")
+
+
 def test_mfma_class_encoding_limits():
     """The int8 MFMA path keeps one class in the low 14 bits of its
     accumulators (at most 16383 samples there).  A class above that puts the
