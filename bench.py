@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--n-var", type=int, default=None, help="override variant count")
     ap.add_argument("--topk", type=int, default=10000)
     ap.add_argument("--ffun", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="mix64 / serve: run the batch autotuner before the timed region")
     ap.add_argument("--count-path", type=int, default=0)
     ap.add_argument("--cli", type=int, default=1,
                     help="large/medium/small: also time the command line end to end")
@@ -121,6 +123,11 @@ def run_mix(args):
         evals += 5 * R * (R + 1) // 2
     gen_s = time.perf_counter() - t
     eng = ge.Engine()
+    tune, tune_s = [], 0.0
+    if args.autotune:
+        t = time.perf_counter()
+        tune = eng.autotune(repeats=2)
+        tune_s = time.perf_counter() - t
 
     def sequential():
         return [it["panel"].scan(it["mask_ctrl"], it["mask_case"], topmost=it["topmost"],
@@ -160,7 +167,9 @@ def run_mix(args):
         "batched_ms": 1000.0 * statistics.mean(bat_t),
         "speedup_vs_sequential": statistics.mean(seq_t) / statistics.mean(bat_t),
         "results_identical_to_sequential": identical,
-        "rounds": bstats[-1]["rounds"],
+        "rounds": bstats[-1]["rounds"], "graph_rounds": bstats[-1]["graph_rounds"],
+        "autotune": {"s": tune_s, "classes": [{k: c[k] for k in ("max_ksteps", "wg_per_cu", "band_evals")}
+                                              for c in tune]} if args.autotune else None,
         "scan_latency_in_batch_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1,
                                                                          int(0.99 * len(lat)))]},
         "roofline": {"bound": "mfma", "kernel": "k_count_f4_batch",
@@ -192,7 +201,9 @@ def run_serve(args):
     specs = synth.mix_specs()
     big_spec = dict(n_var=50000, n_samples=4000, n_case=2000, maf=("uniform", 0.01, 0.5),
                     miss=0.01, planted=0, seed=1005)
-    srv = ge.Server()
+    t = time.perf_counter()
+    srv = ge.Server(autotune=bool(args.autotune))
+    tune_s = time.perf_counter() - t
     mix = []
     for sp in specs:
         p = synth.make_panel(spec=sp)
@@ -201,7 +212,7 @@ def run_serve(args):
         mix.append((pid, m0, m1, p["n_var"], p))
     big = synth.make_panel(spec=big_spec)
     big_id = srv.add_panel_bed(big["payload"], big["n_var"], big["n_samples"])
-    gen_s = time.perf_counter() - t
+    gen_s = time.perf_counter() - t - tune_s
     rng = np.random.Generator(np.random.PCG64(1006))
     arrivals = np.cumsum(rng.exponential(1.0 / args.qps, size=args.requests))
     reqs = []
@@ -265,6 +276,7 @@ def run_serve(args):
         "rounds": st["rounds"] - base["rounds"], "max_active": st["max_active"],
         "failed": st["failed"], "responses_identical_to_alone": identical,
         "roofline": None, "cpu_baseline": None, "gen_s": gen_s,
+        "autotune_s": tune_s if args.autotune else None,
     }
     print(json.dumps(line))
 

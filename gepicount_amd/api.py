@@ -33,6 +33,7 @@ EXPORTS = ("gepi_params_init", "gepi_process_gpu_1", "gepi_panel_create", "gepi_
            "gepi_fisher_pval", "gepi_score_table", "gepi_last_error", "gepi_version",
            "gepi_device_count",
            "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch",
+           "gepi_engine_autotune", "gepi_engine_set_tuning",
            "gepi_server_create", "gepi_server_destroy", "gepi_server_add_panel_bed",
            "gepi_server_release_panel", "gepi_server_submit", "gepi_server_done",
            "gepi_server_wait", "gepi_server_get_stats")
@@ -74,14 +75,24 @@ class BatchItem(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("ms_batch", ctypes.c_double), ("ms_prepare", ctypes.c_double),
                 ("ms_count_kernels", ctypes.c_double), ("evals", ctypes.c_int64),
-                ("rounds", ctypes.c_int32), ("items_failed", ctypes.c_int32)]
+                ("rounds", ctypes.c_int32), ("items_failed", ctypes.c_int32),
+                ("graph_rounds", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneClass(ctypes.Structure):
+    _fields_ = [("max_ksteps", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("band_evals", ctypes.c_double), ("ms_best", ctypes.c_double),
+                ("ms_default", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ServerParams(ctypes.Structure):
-    _fields_ = [("device", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    _fields_ = [("device", ctypes.c_int32), ("autotune", ctypes.c_int32),
                 ("hbm_budget", ctypes.c_int64), ("round_wg_budget", ctypes.c_int64)]
 
 
@@ -162,6 +173,9 @@ def lib():
     l.gepi_engine_destroy.argtypes = [vp]
     l.gepi_engine_scan_batch.argtypes = [vp, ctypes.POINTER(BatchItem), ctypes.c_int32,
                                          ctypes.POINTER(BatchStats)]
+    l.gepi_engine_autotune.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(TuneClass), ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_int32)]
+    l.gepi_engine_set_tuning.argtypes = [vp, ctypes.POINTER(TuneClass), ctypes.c_int32]
     u64 = ctypes.c_uint64
     l.gepi_server_create.argtypes = [ctypes.POINTER(ServerParams), ctypes.POINTER(vp)]
     l.gepi_server_destroy.argtypes = [vp]
@@ -376,6 +390,27 @@ class Engine:
             res.append((out[:arr[i].n_tops].copy(), arr[i].stats.as_dict(), int(arr[i].status)))
         return res, bs.as_dict()
 
+    def autotune(self, repeats=2):
+        """Runs the batch autotuner (gepi_engine_autotune) and returns its
+        table: one dict per size class (max_ksteps, wg_per_cu, band_evals,
+        ms_best, ms_default), ascending max_ksteps."""
+        out = (TuneClass * 16)()
+        n = ctypes.c_int32()
+        _check(lib().gepi_engine_autotune(self._h, int(repeats), out, 16, ctypes.byref(n)))
+        return [out[i].as_dict() for i in range(min(n.value, 16))]
+
+    def set_tuning(self, classes):
+        """Installs a table of size classes (dicts as autotune returns;
+        an empty list restores the defaults)."""
+        arr = (TuneClass * max(len(classes), 1))()
+        for i, c in enumerate(classes):
+            arr[i].max_ksteps = int(c["max_ksteps"])
+            arr[i].wg_per_cu = int(c["wg_per_cu"])
+            arr[i].band_evals = float(c["band_evals"])
+            arr[i].ms_best = float(c.get("ms_best", 0.0))
+            arr[i].ms_default = float(c.get("ms_default", 0.0))
+        _check(lib().gepi_engine_set_tuning(self._h, arr, len(classes)))
+
     def close(self):
         if self._h:
             _check(lib().gepi_engine_destroy(self._h))
@@ -399,8 +434,8 @@ class Server:
     requests admitted between rounds of the batched engine by the server's
     worker thread.  Thread safe."""
 
-    def __init__(self, device=0, hbm_budget=0, round_wg_budget=0):
-        par = ServerParams(int(device), 0, int(hbm_budget), int(round_wg_budget))
+    def __init__(self, device=0, hbm_budget=0, round_wg_budget=0, autotune=False):
+        par = ServerParams(int(device), int(bool(autotune)), int(hbm_budget), int(round_wg_budget))
         h = ctypes.c_void_p()
         _check(lib().gepi_server_create(ctypes.byref(par), ctypes.byref(h)))
         self._h = h

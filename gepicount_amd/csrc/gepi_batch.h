@@ -12,6 +12,12 @@
  *                     the k-th key, in-place compaction to k records, new
  *                     threshold, overflow recovery, a report for the host
  *
+ * A round can also run as two captured HIP graphs (the work-counter clear
+ * and the count launches; the exact and top-k launches) replayed every
+ * round: the grids are then fixed by the descriptor capacity, a scan's
+ * band lives in the descriptor tables, and an empty descriptor slot does
+ * nothing.  k_fill_u32 clears the work counters inside the graph.
+ *
  * There is no counterpart in the reference, which runs one scan per process
  * (src/gEpiCount.cpp:84-87).
  */
@@ -115,14 +121,23 @@ __global__ void __launch_bounds__(F4_THREADS, 2)
     }
 }
 
+/* p[0..n) = v (one workgroup of 64 threads per 64 words). */
+__global__ void __launch_bounds__(64) k_fill_u32(uint32_t *p, int32_t n, uint32_t v) {
+    const int32_t i = (int32_t)(blockIdx.x * 64 + threadIdx.x);
+    if (i < n) p[i] = v;
+}
+
 struct ExactDesc {
     ExactArgs e;
     int32_t ffun;
     int32_t pad;
 };
 
+/* blockIdx.y = descriptor; a zeroed descriptor (no scan: a captured round
+ * launches one row per descriptor slot) does nothing. */
 __global__ void __launch_bounds__(256) k_exact_batch(const ExactDesc *__restrict__ d) {
     const ExactDesc x = d[blockIdx.y];
+    if (x.e.st == nullptr) return;
     const unsigned long long start = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     if (x.ffun == GEPI_FFUNC_ENTROPY)
@@ -183,6 +198,7 @@ __device__ __forceinline__ uint32_t block_prefix(bool flag, uint32_t *wsum, uint
 __global__ void __launch_bounds__(TOPK_THREADS) k_topk_wg(const TopkDesc *__restrict__ dd) {
     const TopkDesc d = dd[blockIdx.x];
     ScanState *st = d.st;
+    if (st == nullptr) return; /* empty descriptor slot (the whole workgroup) */
     __shared__ uint32_t hist[256];
     __shared__ uint64_t kh[SEL_SMALL], kl[SEL_SMALL];
     __shared__ uint32_t wsum[TOPK_THREADS / 64];

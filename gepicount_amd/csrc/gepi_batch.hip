@@ -58,6 +58,61 @@ int Engine::init(int dev) {
     /* fp4 work counters: per objective launch, 2 sets of 8 */
     HIP_TRY(hipMalloc(&d_wq_, sizeof(uint32_t) * 32));
     HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
+    const char *g = getenv("GEPI_ENGINE_GRAPH");
+    use_graph_ = !(g && atoi(g) == 0);
+    return GEPI_OK;
+}
+
+void Engine::drop_graphs() {
+    for (auto &g : gx_count_) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    if (gx_finish_) hipGraphExecDestroy(gx_finish_);
+    gx_finish_ = nullptr;
+    gx_cap_ = 0;
+    gx_per_cu_ = 0;
+}
+
+int Engine::capture_graph(int part, int per_cu) {
+    const size_t cap = desc_cap_;
+    const unsigned grid =
+        (unsigned)std::max<int64_t>(8, (per_cu * (int64_t)device_cus(device)) / 8 * 8);
+    /* nothing may return between the begin and the end of the capture */
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    if (part < 4) {
+        k_fill_u32<<<1, 64, 0, stream>>>(d_wq_, 32, 0u);
+        for (int f = 0; f < 2; f++) {
+            if (!(part & (1 << f))) continue;
+            /* the maps are laid out for cap scans; the slots past a round's
+             * scans have empty ranges                                       */
+            BatchMap m;
+            m.xprefix = d_map_[f];
+            m.xstart = d_map_[f] + 8 * (cap + 1);
+            m.n = (int32_t)cap;
+            m.wq = d_wq_ + 16 * f;
+            m.wq_parity = 0;
+            if (f == 0)
+                k_count_f4_batch<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+            else
+                k_count_f4_batch<GEPI_FFUNC_FISHERPV, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+        }
+    } else {
+        k_exact_batch<<<dim3(EXACT_BLOCKS_X, (unsigned)cap), 256, 0, stream>>>(d_edesc_);
+        k_topk_wg<<<(unsigned)cap, TOPK_THREADS, 0, stream>>>(d_tdesc_);
+    }
+    const hipError_t le = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(stream, &graph);
+    hipError_t ie = hipSuccess;
+    if (le == hipSuccess && ce == hipSuccess)
+        ie = hipGraphInstantiate(part < 4 ? &gx_count_[part] : &gx_finish_, graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (le != hipSuccess || ce != hipSuccess || ie != hipSuccess) {
+        const hipError_t e = le != hipSuccess ? le : ce != hipSuccess ? ce : ie;
+        drop_graphs();
+        return gepi_fail(GEPI_ERR_DEVICE, "round graph capture: %s", hipGetErrorString(e));
+    }
     return GEPI_OK;
 }
 
@@ -67,6 +122,7 @@ void Engine::release() {
         hipSetDevice(device);
         hipStreamSynchronize(stream);
     }
+    drop_graphs();
     for (ScanItem *it : active_) drop(it, GEPI_ERR_INTERNAL);
     active_.clear();
     for (auto &b : free_) hipFree(b.second);
@@ -131,9 +187,136 @@ void Engine::dfree(void *p) {
     live_.erase(it);
 }
 
+const gepi_tune_class *Engine::tune_of(int32_t n_k4) const {
+    if (tuning.empty()) return nullptr;
+    for (const gepi_tune_class &c : tuning)
+        if (n_k4 <= c.max_ksteps) return &c;
+    return &tuning.back();
+}
+
+int Engine::time_batch(std::vector<ScanItem> &items, double *ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    for (ScanItem &it : items) add(&it);
+    std::vector<ScanItem *> fin;
+    while (!idle())
+        if ((rc = round(fin)) != GEPI_OK) {
+            /* the items are the caller's: none may stay active */
+            for (ScanItem *it : active_) drop(it, rc);
+            active_.clear();
+            return rc;
+        }
+    *ms = ms_since(t0);
+    for (const ScanItem &it : items)
+        if (it.status != GEPI_OK) return it.status;
+    return GEPI_OK;
+}
+
+namespace {
+
+/* Synthetic SNP-major .bed bytes: per variant an allele frequency in
+ * [0.05, 0.5], genotypes by Hardy-Weinberg, 1% missing calls (xorshift64*). */
+std::vector<uint8_t> synth_bed(int64_t n_var, int32_t n, uint64_t seed) {
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&s]() {
+        s ^= s >> 12;
+        s ^= s << 25;
+        s ^= s >> 27;
+        return s * 2685821657736338717ull;
+    };
+    auto unif = [&next]() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); };
+    const int64_t bpv = (n + 3) / 4;
+    std::vector<uint8_t> bed((size_t)(n_var * bpv), 0);
+    for (int64_t v = 0; v < n_var; v++) {
+        const double f = 0.05 + 0.45 * unif();
+        const double p0 = (1 - f) * (1 - f), p1 = p0 + 2 * f * (1 - f);
+        for (int32_t i = 0; i < n; i++) {
+            const double u = unif();
+            /* .bed codes: 0 homozygous A1, 2 heterozygous, 3 homozygous A2, 1 missing */
+            const uint8_t code = unif() < 0.01 ? 1 : u < p0 ? 0 : u < p1 ? 2 : 3;
+            bed[(size_t)(v * bpv + i / 4)] |= (uint8_t)(code << (2 * (i % 4)));
+        }
+    }
+    return bed;
+}
+
+} // namespace
+
+int Engine::autotune(int32_t repeats) {
+    struct Trial {
+        int32_t max_k4;
+        int64_t n_var;
+        int32_t n;
+    };
+    const Trial trials[4] = {{2, 16000, 256}, {8, 10000, 1024}, {32, 8000, 4096}, {INT32_MAX, 6000, 8192}};
+    const int per_cus[2] = {1, 2};
+    const double bands[3] = {5e8, 2e9, 8e9};
+    repeats = std::max(1, std::min(repeats, 16));
+    std::vector<gepi_tune_class> table;
+    int rc = GEPI_OK;
+    tuning.clear();
+    for (int c = 0; c < 4 && rc == GEPI_OK; c++) {
+        const Trial &t = trials[c];
+        const std::vector<uint8_t> bed = synth_bed(t.n_var, t.n, 1000 + c);
+        gepi_panel *p = nullptr;
+        if ((rc = gepi_panel_create_bed(bed.data(), t.n_var, t.n, device, &p)) != GEPI_OK) break;
+        /* 4 phenotypes of about half cases each */
+        const int32_t w64 = (t.n + 63) / 64;
+        std::vector<std::vector<uint64_t>> m0(4, std::vector<uint64_t>((size_t)w64, 0)),
+            m1(4, std::vector<uint64_t>((size_t)w64, 0));
+        uint64_t s = 77 + (uint64_t)c;
+        for (int j = 0; j < 4; j++)
+            for (int32_t i = 0; i < t.n; i++) {
+                s = s * 6364136223846793005ull + 1442695040888963407ull;
+                ((s >> 40) & 1 ? m1 : m0)[(size_t)j][(size_t)(i / 64)] |= 1ull << (i % 64);
+            }
+        std::vector<ScanItem> items(4);
+        for (int j = 0; j < 4; j++) {
+            items[(size_t)j].panel = p;
+            items[(size_t)j].mask_ctrl = m0[(size_t)j].data();
+            items[(size_t)j].mask_case = m1[(size_t)j].data();
+            items[(size_t)j].topmost = 100;
+            items[(size_t)j].ffun = GEPI_FFUNC_ENTROPY;
+        }
+        gepi_tune_class best{t.max_k4, 2, 2e9, 0.0, 0.0};
+        double ms_best = INFINITY, ms_dflt = INFINITY, ms;
+        /* warm-up (allocations, tables, graphs) under the default */
+        tuning.assign(1, gepi_tune_class{INT32_MAX, 2, 2e9, 0.0, 0.0});
+        rc = time_batch(items, &ms);
+        for (int r = 0; r < repeats && rc == GEPI_OK; r++)
+            for (int pc : per_cus)
+                for (double b : bands) {
+                    if (rc != GEPI_OK) break;
+                    tuning.assign(1, gepi_tune_class{INT32_MAX, pc, b, 0.0, 0.0});
+                    if ((rc = time_batch(items, &ms)) != GEPI_OK) break;
+                    if (pc == 2 && b == 2e9) ms_dflt = std::min(ms_dflt, ms);
+                    /* the best of each schedule over the repeats */
+                    if (ms < ms_best) {
+                        ms_best = ms;
+                        best.wg_per_cu = pc;
+                        best.band_evals = b;
+                    }
+                }
+        gepi_panel_destroy(p);
+        if (rc != GEPI_OK) break;
+        if (ms_dflt <= 1.03 * ms_best) {
+            /* within the trial's noise: keep the default */
+            best.wg_per_cu = 2;
+            best.band_evals = 2e9;
+            ms_best = ms_dflt;
+        }
+        best.ms_best = ms_best;
+        best.ms_default = ms_dflt;
+        table.push_back(best);
+    }
+    tuning = rc == GEPI_OK ? table : std::vector<gepi_tune_class>();
+    return rc;
+}
+
 int Engine::grow_desc(size_t n) {
     if (n <= desc_cap_) return GEPI_OK;
     const size_t c = std::max<size_t>(n, 2 * desc_cap_);
+    drop_graphs(); /* they hold the old tables' addresses */
     for (int i = 0; i < 2; i++) {
         hipFree(d_bdesc_[i]);
         hipFree(d_map_[i]);
@@ -286,7 +469,9 @@ int Engine::add(ScanItem *it) {
             it->d_layout);
         HIP_TRY(hipGetLastError());
     }
-    it->max_band = std::max<int64_t>(64, (int64_t)(2.0e9 / (double)F4_TILE_EVALS));
+    const gepi_tune_class *tc = tune_of(it->n_k4);
+    const double band_evals = tc ? tc->band_evals : 2.0e9;
+    it->max_band = std::max<int64_t>(64, (int64_t)(band_evals / (double)F4_TILE_EVALS));
     it->band = std::max<int64_t>(1, std::min<int64_t>(it->max_band, (2 * it->k) / F4_TILE_EVALS + 1));
     it->forced_band = 0;
     if (const char *env = getenv("GEPI_BAND_TILES")) {
@@ -365,14 +550,29 @@ int Engine::round(std::vector<ScanItem *> &finished) {
     /* per objective: descriptors with the longest K loops first (their
      * workgroups are dispatched first; short ones fill the tail), and the
      * per-XCD chunk tables of BatchMap */
+    /* launch form: the one of the class with the most count work (tiles x
+     * K steps) in this round                                               */
+    int per_cu_dflt = 2;
+    if (!tuning.empty()) {
+        std::vector<double> work(tuning.size(), 0.0);
+        for (ScanItem *it : active_)
+            work[(size_t)(tune_of(it->n_k4) - tuning.data())] +=
+                (double)it->band_used * (double)std::max(it->n_k4, 1);
+        const size_t c = (size_t)(std::max_element(work.begin(), work.end()) - work.begin());
+        per_cu_dflt = tuning[c].wg_per_cu;
+    }
+    const int per_cu = f4_wg_per_cu(per_cu_dflt);
+    /* captured rounds need a fixed grid: the persistent launch form */
+    const bool graph = use_graph_ && per_cu > 0;
     std::vector<int64_t> map[2];
     for (int f = 0; f < 2; f++) {
         std::vector<ScanItem *> &g = grp[f];
         std::stable_sort(g.begin(), g.end(),
                          [](const ScanItem *a, const ScanItem *b) { return a->n_k4 > b->n_k4; });
         const size_t n = g.size();
-        map[f].assign(8 * (n + 1) + 8 * n, 0);
-        int64_t *xpre = map[f].data(), *xst = map[f].data() + 8 * (n + 1);
+        const size_t ns = graph ? desc_cap_ : n; /* scan slots of the map */
+        map[f].assign(8 * (ns + 1) + 8 * ns, 0);
+        int64_t *xpre = map[f].data(), *xst = map[f].data() + 8 * (ns + 1);
         int64_t dealt = 0;
         for (size_t j = 0; j < n; j++) {
             ScanItem *it = g[j];
@@ -397,67 +597,94 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             int64_t start = 0;
             for (int x = 0; x < 8; x++) {
                 const int64_t chunk = b / 8 + (((x - r) & 7) < (b & 7) ? 1 : 0);
-                xst[(size_t)x * n + j] = start;
-                xpre[(size_t)x * (n + 1) + j + 1] = xpre[(size_t)x * (n + 1) + j] + chunk;
+                xst[(size_t)x * ns + j] = start;
+                xpre[(size_t)x * (ns + 1) + j + 1] = xpre[(size_t)x * (ns + 1) + j] + chunk;
                 start += chunk;
             }
             dealt += b;
         }
+        /* slots past the round's scans: empty ranges */
+        for (int x = 0; x < 8; x++)
+            for (size_t j = n; j < ns; j++)
+                xpre[(size_t)x * (ns + 1) + j + 1] = xpre[(size_t)x * (ns + 1) + n];
         wg[f] = dealt;
     }
-    for (int f = 0; f < 2; f++)
-        if (!bd[f].empty()) {
+    if (graph) {
+        /* every descriptor slot is launched: the ones past the round's scans
+         * are zeroed (no scan)                                              */
+        ed.resize(desc_cap_);
+        td.resize(desc_cap_);
+    }
+    for (int f = 0; f < 2; f++) {
+        if (!bd[f].empty())
             HIP_TRY(hipMemcpyAsync(d_bdesc_[f], bd[f].data(), sizeof(BatchDesc) * bd[f].size(),
                                    hipMemcpyHostToDevice, stream));
+        if (!bd[f].empty() || graph)
             HIP_TRY(hipMemcpyAsync(d_map_[f], map[f].data(), sizeof(int64_t) * map[f].size(),
                                    hipMemcpyHostToDevice, stream));
-        }
-    HIP_TRY(hipMemcpyAsync(d_edesc_, ed.data(), sizeof(ExactDesc) * na, hipMemcpyHostToDevice,
-                           stream));
-    HIP_TRY(hipMemcpyAsync(d_tdesc_, td.data(), sizeof(TopkDesc) * na, hipMemcpyHostToDevice,
-                           stream));
-    /* work counters start clean every round (a failed round leaves no
-     * stale counts behind) */
-    HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
-    wq_parity_[0] = wq_parity_[1] = 0;
-    HIP_TRY(hipEventRecord(ev0_, stream));
-    for (int f = 0; f < 2; f++) {
-        if (wg[f] == 0) continue;
-        const size_t n = bd[f].size();
-        BatchMap m;
-        m.xprefix = d_map_[f];
-        m.xstart = d_map_[f] + 8 * (n + 1);
-        m.n = (int32_t)n;
-        m.wq = d_wq_ + 16 * f;
-        m.wq_parity = wq_parity_[f];
-        const int per_cu = f4_wg_per_cu(2);
-        /* one slot per workgroup: the grid covers the largest XCD share */
-        int64_t xmax = 0;
-        for (int x = 0; x < 8; x++) xmax = std::max<int64_t>(xmax, map[f][(size_t)x * (n + 1) + n]);
-        unsigned grid = (unsigned)(8 * xmax);
-        if (per_cu > 0) {
-            wq_parity_[f] ^= 1;
-            grid = (unsigned)std::min<int64_t>(
-                std::max<int64_t>(8, (per_cu * (int64_t)device_cus(device)) / 8 * 8),
-                ((wg[f] + 7) / 8) * 8);
-        }
-        if (f == 0) {
-            if (per_cu > 0)
-                k_count_f4_batch<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-            else
-                k_count_f4_batch<GEPI_FFUNC_ENTROPY, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-        } else {
-            if (per_cu > 0)
-                k_count_f4_batch<GEPI_FFUNC_FISHERPV, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-            else
-                k_count_f4_batch<GEPI_FFUNC_FISHERPV, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-        }
     }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ev1_, stream));
-    k_exact_batch<<<dim3(EXACT_BLOCKS_X, (unsigned)na), 256, 0, stream>>>(d_edesc_);
-    k_topk_wg<<<(unsigned)na, TOPK_THREADS, 0, stream>>>(d_tdesc_);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d_edesc_, ed.data(), sizeof(ExactDesc) * ed.size(), hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipMemcpyAsync(d_tdesc_, td.data(), sizeof(TopkDesc) * td.size(), hipMemcpyHostToDevice,
+                           stream));
+    wq_parity_[0] = wq_parity_[1] = 0;
+    if (graph) {
+        if (gx_cap_ != desc_cap_ || gx_per_cu_ != per_cu) {
+            drop_graphs();
+            gx_cap_ = desc_cap_;
+            gx_per_cu_ = per_cu;
+        }
+        const int mask = (wg[0] > 0 ? 1 : 0) | (wg[1] > 0 ? 2 : 0);
+        if (!gx_count_[mask] && (rc = capture_graph(mask, per_cu)) != GEPI_OK) return rc;
+        if (!gx_finish_ && (rc = capture_graph(4, per_cu)) != GEPI_OK) return rc;
+        /* the count graph clears the work counters first */
+        HIP_TRY(hipEventRecord(ev0_, stream));
+        HIP_TRY(hipGraphLaunch(gx_count_[mask], stream));
+        HIP_TRY(hipEventRecord(ev1_, stream));
+        HIP_TRY(hipGraphLaunch(gx_finish_, stream));
+        graph_rounds++;
+    } else {
+        /* work counters start clean every round (a failed round leaves no
+         * stale counts behind) */
+        HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
+        HIP_TRY(hipEventRecord(ev0_, stream));
+        for (int f = 0; f < 2; f++) {
+            if (wg[f] == 0) continue;
+            const size_t n = bd[f].size();
+            BatchMap m;
+            m.xprefix = d_map_[f];
+            m.xstart = d_map_[f] + 8 * (n + 1);
+            m.n = (int32_t)n;
+            m.wq = d_wq_ + 16 * f;
+            m.wq_parity = wq_parity_[f];
+            /* one slot per workgroup: the grid covers the largest XCD share */
+            int64_t xmax = 0;
+            for (int x = 0; x < 8; x++) xmax = std::max<int64_t>(xmax, map[f][(size_t)x * (n + 1) + n]);
+            unsigned grid = (unsigned)(8 * xmax);
+            if (per_cu > 0) {
+                wq_parity_[f] ^= 1;
+                grid = (unsigned)std::min<int64_t>(
+                    std::max<int64_t>(8, (per_cu * (int64_t)device_cus(device)) / 8 * 8),
+                    ((wg[f] + 7) / 8) * 8);
+            }
+            if (f == 0) {
+                if (per_cu > 0)
+                    k_count_f4_batch<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+                else
+                    k_count_f4_batch<GEPI_FFUNC_ENTROPY, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+            } else {
+                if (per_cu > 0)
+                    k_count_f4_batch<GEPI_FFUNC_FISHERPV, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+                else
+                    k_count_f4_batch<GEPI_FFUNC_FISHERPV, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+            }
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ev1_, stream));
+        k_exact_batch<<<dim3(EXACT_BLOCKS_X, (unsigned)na), 256, 0, stream>>>(d_edesc_);
+        k_topk_wg<<<(unsigned)na, TOPK_THREADS, 0, stream>>>(d_tdesc_);
+        HIP_TRY(hipGetLastError());
+    }
     std::vector<ItemReport> rep(na);
     HIP_TRY(hipMemcpyAsync(rep.data(), d_rep_, sizeof(ItemReport) * na, hipMemcpyDeviceToHost,
                            stream));
@@ -542,6 +769,37 @@ extern "C" int gepi_engine_destroy(gepi_engine *e) {
     return GEPI_OK;
 }
 
+extern "C" int gepi_engine_autotune(gepi_engine *e, int32_t repeats, gepi_tune_class *out,
+                                    int32_t n_out, int32_t *n_classes) {
+    if (!e) return gepi_fail(GEPI_ERR_ARG, "null engine");
+    if (n_out < 0 || (n_out > 0 && !out)) return gepi_fail(GEPI_ERR_ARG, "bad output table");
+    if (!e->eng.idle()) return gepi_fail(GEPI_ERR_ARG, "engine busy");
+    int rc;
+    if ((rc = set_device(e->eng.device)) != GEPI_OK) return rc;
+    if ((rc = e->eng.autotune(repeats)) != GEPI_OK) return rc;
+    const int32_t n = (int32_t)e->eng.tuning.size();
+    for (int32_t i = 0; i < std::min(n, n_out); i++) out[i] = e->eng.tuning[(size_t)i];
+    if (n_classes) *n_classes = n;
+    return GEPI_OK;
+}
+
+extern "C" int gepi_engine_set_tuning(gepi_engine *e, const gepi_tune_class *classes, int32_t n) {
+    if (!e) return gepi_fail(GEPI_ERR_ARG, "null engine");
+    if (n < 0 || n > 16 || (n > 0 && !classes)) return gepi_fail(GEPI_ERR_ARG, "bad class count %d", n);
+    for (int32_t i = 0; i < n; i++) {
+        const gepi_tune_class &c = classes[i];
+        if ((i > 0 && c.max_ksteps <= classes[i - 1].max_ksteps) || c.max_ksteps < 1)
+            return gepi_fail(GEPI_ERR_ARG, "class %d: max_ksteps must increase", i);
+        if (c.wg_per_cu < 1 || c.wg_per_cu > 16)
+            return gepi_fail(GEPI_ERR_ARG, "class %d: wg_per_cu %d outside [1, 16]", i, c.wg_per_cu);
+        if (!(c.band_evals >= 1e6) || !std::isfinite(c.band_evals))
+            return gepi_fail(GEPI_ERR_ARG, "class %d: band_evals below 1e6", i);
+    }
+    if (!e->eng.idle()) return gepi_fail(GEPI_ERR_ARG, "engine busy");
+    e->eng.tuning.assign(classes, classes + n);
+    return GEPI_OK;
+}
+
 extern "C" int gepi_engine_scan_batch(gepi_engine *e, gepi_batch_item *items, int32_t n_items,
                                       gepi_batch_stats *stats) {
     if (!e) return gepi_fail(GEPI_ERR_ARG, "null engine");
@@ -554,6 +812,7 @@ extern "C" int gepi_engine_scan_batch(gepi_engine *e, gepi_batch_item *items, in
     std::vector<ScanItem> its((size_t)n_items);
     e->eng.ms_count = 0.0;
     e->eng.rounds = 0;
+    e->eng.graph_rounds = 0;
     for (int32_t i = 0; i < n_items; i++) {
         ScanItem &it = its[(size_t)i];
         it.panel = items[i].panel;
@@ -593,6 +852,7 @@ extern "C" int gepi_engine_scan_batch(gepi_engine *e, gepi_batch_item *items, in
     }
     bs.ms_count_kernels = e->eng.ms_count;
     bs.rounds = e->eng.rounds;
+    bs.graph_rounds = e->eng.graph_rounds;
     bs.ms_batch = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
                       .count();
     if (stats) *stats = bs;

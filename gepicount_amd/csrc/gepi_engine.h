@@ -2,7 +2,8 @@
  * gepi_engine.h -- the batched scan engine (host side).  Scans are added at
  * any time and advance together in rounds; a round is one count launch per
  * objective, one exact-score launch and one top-k launch for all active
- * scans, then a single read of their reports.  The batch C ABI runs rounds
+ * scans, then a single read of their reports; by default a round's launches
+ * are replayed from two captured HIP graphs.  The batch C ABI runs rounds
  * until every scan is done; the serving loop admits new requests between
  * rounds.  All device work runs on the engine's stream.
  */
@@ -75,8 +76,14 @@ class Engine {
     int64_t round_wg_budget = 0;
     double ms_count = 0.0;
     int32_t rounds = 0;
+    int32_t graph_rounds = 0; /* rounds replayed from the captured graphs     */
     int device = 0;
     hipStream_t stream = nullptr;
+    /* Schedule per size class, ascending max_ksteps (empty: 2 workgroups
+     * per CU and 2e9 evaluations per band for every scan).                  */
+    std::vector<gepi_tune_class> tuning;
+    /* Measures the table on synthetic batches (gepi_engine_autotune). */
+    int autotune(int32_t repeats);
 
   private:
     void *dalloc(size_t bytes);
@@ -84,6 +91,18 @@ class Engine {
     int finish(ScanItem *it);
     void drop(ScanItem *it, int status);
     int grow_desc(size_t n);
+    /* The class of a scan with n_k4 K steps (nullptr without a table). */
+    const gepi_tune_class *tune_of(int32_t n_k4) const;
+    /* Wall time (ms) of one synthetic batch: every item added, rounds
+     * until idle.                                                           */
+    int time_batch(std::vector<ScanItem> &items, double *ms);
+    /* Captures part of a round for descriptor capacity desc_cap_ and a
+     * persistent grid of per_cu workgroups per CU: part 0-3 into
+     * gx_count_[part] (work-counter clear, then the count launches of the
+     * objectives in bit mask part: 1 entropy, 2 Fisher), part 4 into
+     * gx_finish_ (exact + top-k launches).                                  */
+    int capture_graph(int part, int per_cu);
+    void drop_graphs();
 
     DeviceTables tabs_;
     std::vector<ScanItem *> active_;
@@ -99,6 +118,13 @@ class Engine {
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t *d_wq_ = nullptr; /* fp4 work counters: [objective][2 sets][8] */
     int32_t wq_parity_[2] = {0, 0};
+    /* rounds as captured graphs (GEPI_ENGINE_GRAPH=0 launches them one by
+     * one); valid for gx_cap_ descriptor slots and gx_per_cu_ workgroups
+     * per CU                                                                 */
+    bool use_graph_ = true;
+    hipGraphExec_t gx_count_[4] = {nullptr, nullptr, nullptr, nullptr}, gx_finish_ = nullptr;
+    size_t gx_cap_ = 0;
+    int gx_per_cu_ = 0;
 };
 
 } // namespace gepi_host

@@ -194,6 +194,10 @@ extern "C" int gepi_server_create(const gepi_server_params *par, gepi_server **o
         return rc;
     }
     s->eng.round_wg_budget = p.round_wg_budget > 0 ? p.round_wg_budget : 262144;
+    if (p.autotune && (rc = s->eng.autotune(2)) != GEPI_OK) {
+        s->eng.release();
+        return rc;
+    }
     s->budget = p.hbm_budget;
     if (s->budget <= 0) {
         size_t fr = 0, tot = 0;

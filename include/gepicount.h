@@ -182,10 +182,39 @@ typedef struct gepi_batch_stats {
     int64_t evals;              /* summed over items                            */
     int32_t rounds;             /* (count, exact, top-k) rounds                 */
     int32_t items_failed;
+    int32_t graph_rounds;       /* rounds replayed from captured HIP graphs     */
+    int32_t reserved;
 } gepi_batch_stats;
 
 int gepi_engine_create(int32_t device, gepi_engine **out);
 int gepi_engine_destroy(gepi_engine *engine);
+
+/* Schedule of the batched engine for one size class: scans with at most
+ * max_ksteps 128-sample K steps (both classes together; the last class
+ * takes every larger scan).  No counterpart in the reference.              */
+typedef struct gepi_tune_class {
+    int32_t max_ksteps;
+    int32_t wg_per_cu;   /* persistent count workgroups per CU                */
+    double band_evals;   /* evaluations per scan and round                    */
+    double ms_best;      /* autotune trial of the class: chosen schedule      */
+    double ms_default;   /* the same trial with 2 workgroups per CU, 2e9      */
+} gepi_tune_class;
+
+/* Autotuner: for each of 4 size classes (at most 2, 8, 32 and any number of
+ * K steps), runs a synthetic batch of 4 scans of that class (16,000 variants
+ * x 256 samples, 10,000 x 1,024, 8,000 x 4,096, 6,000 x 8,192; top-100)
+ * under every candidate schedule (1 or 2 workgroups per CU x 5e8, 2e9 or
+ * 8e9 evaluations per band), repeats times each, and keeps the fastest
+ * (the default unless another is more than 3% faster).  The engine uses the
+ * table from then on: a scan's band size by its class, a round's launch
+ * form by the class with the most count work in it.  Results never depend
+ * on the schedule.  Writes min(n_out, 4) classes; *n_classes = 4.          */
+int gepi_engine_autotune(gepi_engine *engine, int32_t repeats, gepi_tune_class *out,
+                         int32_t n_out, int32_t *n_classes);
+/* Installs a table (from an earlier autotune): 1-16 classes, max_ksteps
+ * strictly increasing, wg_per_cu in [1, 16], band_evals >= 1e6.  n = 0
+ * restores the defaults.                                                     */
+int gepi_engine_set_tuning(gepi_engine *engine, const gepi_tune_class *classes, int32_t n);
 /* Runs every item to completion.  Returns GEPI_OK when the batch ran (an
  * item's own failure is in its status) or the device error that stopped it. */
 int gepi_engine_scan_batch(gepi_engine *engine, gepi_batch_item *items, int32_t n_items,
@@ -205,7 +234,9 @@ typedef struct gepi_server gepi_server;
 
 typedef struct gepi_server_params {
     int32_t device;
-    int32_t reserved;
+    int32_t autotune;        /* nonzero: the server's engine runs the batch
+                                autotuner (gepi_engine_autotune, 2 repeats)
+                                before the first request                      */
     int64_t hbm_budget;      /* scan workspace admitted at once, bytes; 0 = 3/4
                                 of the device memory free at creation; a
                                 request larger than the budget runs alone      */

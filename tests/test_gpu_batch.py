@@ -183,3 +183,83 @@ def test_batch_everything_survives_early():
         ref = orc.gold_scan(orc.Panel(bits, valid, items[i]["mask_ctrl"], items[i]["mask_case"], n),
                             items[i]["topmost"])
         assert_same(got, ref, "item %d vs oracle" % i)
+
+
+def run_batch(panels, items, engine):
+    handles = [ge.Panel.from_planes(b, v, n) for b, v, n in panels]
+    try:
+        return engine.scan_batch([dict(it, panel=handles[it["panel_index"]]) for it in items])
+    finally:
+        for h in handles:
+            h.close()
+
+
+def test_batch_graph_replays_equal_eager(monkeypatch):
+    """Rounds replayed from the captured graphs (the default) against rounds
+    launched one by one (GEPI_ENGINE_GRAPH=0): three batches in a row on one
+    engine (6 items; then 12, which grows the descriptor tables and captures
+    again; then 4, which leaves descriptor slots empty).  Every round is a
+    replay, with the host's uploads and report reads between replays; each
+    item must equal the eager run's result and the oracle's."""
+    print("case batch graph replays", flush=True)
+    mixes = [make_mix(s, n, vmax=260, nmax=900) for s, n in ((31, 6), (32, 12), (33, 4))]
+    monkeypatch.delenv("GEPI_F4_WG_PER_CU", raising=False)
+    monkeypatch.setenv("GEPI_ENGINE_GRAPH", "0")
+    eager = []
+    with ge.Engine() as eng:
+        for panels, items in mixes:
+            res, bs = run_batch(panels, items, eng)
+            assert bs["graph_rounds"] == 0 and bs["items_failed"] == 0
+            eager.append(res)
+    monkeypatch.delenv("GEPI_ENGINE_GRAPH")
+    with ge.Engine() as eng:
+        for j, (panels, items) in enumerate(mixes):
+            res, bs = run_batch(panels, items, eng)
+            assert bs["items_failed"] == 0
+            assert bs["graph_rounds"] == bs["rounds"] >= 2, bs
+            for i, it in enumerate(items):
+                got, st, status = res[i]
+                assert status == 0, (j, i, status)
+                assert_same(got, eager[j][i][0], "batch %d item %d graph vs eager" % (j, i))
+                bits, valid, n = panels[it["panel_index"]]
+                ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], n),
+                                    it["topmost"], ffun=it["ffun"])
+                assert_same(got, ref, "batch %d item %d vs oracle" % (j, i))
+
+
+def test_batch_autotune_and_tables(monkeypatch):
+    """gepi_engine_autotune returns one schedule per size class from its
+    candidate set; batches under the tuned table and under a hand-set table
+    of extreme schedules (1 workgroup per CU and 64-tile bands for short
+    scans) equal the default engine's results and the oracle's; bad tables
+    are rejected and leave the engine usable."""
+    print("case batch autotune", flush=True)
+    monkeypatch.delenv("GEPI_F4_WG_PER_CU", raising=False)
+    panels, items = make_mix(41, 8, vmax=900, nmax=1500)
+    with ge.Engine() as eng:
+        base, bs0 = run_batch(panels, items, eng)
+        table = eng.autotune(repeats=1)
+        print("autotune table", table, flush=True)
+        assert [c["max_ksteps"] for c in table] == [2, 8, 32, 2**31 - 1]
+        for c in table:
+            assert c["wg_per_cu"] in (1, 2) and c["band_evals"] in (5e8, 2e9, 8e9), c
+            assert 0 < c["ms_best"] <= c["ms_default"], c
+        tuned, _ = run_batch(panels, items, eng)
+        eng.set_tuning([dict(max_ksteps=8, wg_per_cu=1, band_evals=1e6),
+                        dict(max_ksteps=2**31 - 1, wg_per_cu=2, band_evals=3e6)])
+        extreme, bs2 = run_batch(panels, items, eng)
+        assert bs2["rounds"] >= bs0["rounds"]
+        for bad in ([dict(max_ksteps=8, wg_per_cu=2, band_evals=2e9)] * 2,
+                    [dict(max_ksteps=8, wg_per_cu=0, band_evals=2e9)],
+                    [dict(max_ksteps=8, wg_per_cu=2, band_evals=10.0)]):
+            with pytest.raises(ge.GepiError):
+                eng.set_tuning(bad)
+        eng.set_tuning([])
+        again, _ = run_batch(panels, items, eng)
+    for i, it in enumerate(items):
+        bits, valid, n = panels[it["panel_index"]]
+        ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], n),
+                            it["topmost"], ffun=it["ffun"])
+        for label, res in (("default", base), ("tuned", tuned), ("extreme", extreme), ("reset", again)):
+            assert res[i][2] == 0, (label, i, res[i][2])
+            assert_same(res[i][0], ref, "%s item %d vs oracle" % (label, i))

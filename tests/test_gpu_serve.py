@@ -148,3 +148,17 @@ def test_server_errors_and_panel_release():
         assert srv.stats()["panels"] == 0
         with pytest.raises(ge.GepiError):
             srv.release_panel(pid)
+
+
+def test_server_autotuned():
+    """A server whose engine ran the autotuner at creation serves results
+    identical to the oracle's."""
+    print("case server autotuned", flush=True)
+    panels = make_panels(13, 3)
+    reqs = make_requests(14, panels, 9)
+    with ge.Server(autotune=True) as srv:
+        ids = [srv.add_panel_bed(p["payload"], p["n_var"], p["n"]) for p in panels]
+        tickets = [srv.submit(ids[r["pi"]], r["m0"], r["m1"], topmost=r["k"], ffun=r["ffun"])
+                   for r in reqs]
+        got = [srv.wait(t) for t in tickets]
+    check_against_alone_and_oracle(panels, reqs, got)
