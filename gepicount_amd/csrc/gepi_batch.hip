@@ -58,8 +58,10 @@ int Engine::init(int dev) {
     /* fp4 work counters: per objective launch, 2 sets of 8 */
     HIP_TRY(hipMalloc(&d_wq_, sizeof(uint32_t) * 32));
     HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
+    /* rounds as captured graphs only on request: on the MI355X a graph
+     * launch costs about 0.16 ms per round more than the direct launches */
     const char *g = getenv("GEPI_ENGINE_GRAPH");
-    use_graph_ = !(g && atoi(g) == 0);
+    use_graph_ = g && atoi(g) != 0;
     return GEPI_OK;
 }
 
@@ -690,7 +692,11 @@ int Engine::round(std::vector<ScanItem *> &finished) {
                            stream));
     HIP_TRY(hipStreamSynchronize(stream));
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, ev0_, ev1_);
+    if (hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess) {
+        /* a statistic only: leave no error behind for the next call */
+        ms = 0.0f;
+        (void)hipGetLastError();
+    }
     ms_count += ms;
     rounds++;
 

@@ -2,8 +2,9 @@
  * gepi_engine.h -- the batched scan engine (host side).  Scans are added at
  * any time and advance together in rounds; a round is one count launch per
  * objective, one exact-score launch and one top-k launch for all active
- * scans, then a single read of their reports; by default a round's launches
- * are replayed from two captured HIP graphs.  The batch C ABI runs rounds
+ * scans, then a single read of their reports; with GEPI_ENGINE_GRAPH=1 a
+ * round's launches are replayed from two captured HIP graphs.  The batch C
+ * ABI runs rounds
  * until every scan is done; the serving loop admits new requests between
  * rounds.  All device work runs on the engine's stream.
  */
@@ -118,10 +119,10 @@ class Engine {
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t *d_wq_ = nullptr; /* fp4 work counters: [objective][2 sets][8] */
     int32_t wq_parity_[2] = {0, 0};
-    /* rounds as captured graphs (GEPI_ENGINE_GRAPH=0 launches them one by
+    /* rounds as captured graphs (GEPI_ENGINE_GRAPH=1; else launched one by
      * one); valid for gx_cap_ descriptor slots and gx_per_cu_ workgroups
      * per CU                                                                 */
-    bool use_graph_ = true;
+    bool use_graph_ = false;
     hipGraphExec_t gx_count_[4] = {nullptr, nullptr, nullptr, nullptr}, gx_finish_ = nullptr;
     size_t gx_cap_ = 0;
     int gx_per_cu_ = 0;

@@ -195,8 +195,8 @@ def run_batch(panels, items, engine):
 
 
 def test_batch_graph_replays_equal_eager(monkeypatch):
-    """Rounds replayed from the captured graphs (the default) against rounds
-    launched one by one (GEPI_ENGINE_GRAPH=0): three batches in a row on one
+    """Rounds replayed from the captured graphs (GEPI_ENGINE_GRAPH=1) against
+    rounds launched one by one (the default): three batches in a row on one
     engine (6 items; then 12, which grows the descriptor tables and captures
     again; then 4, which leaves descriptor slots empty).  Every round is a
     replay, with the host's uploads and report reads between replays; each
@@ -211,7 +211,7 @@ def test_batch_graph_replays_equal_eager(monkeypatch):
             res, bs = run_batch(panels, items, eng)
             assert bs["graph_rounds"] == 0 and bs["items_failed"] == 0
             eager.append(res)
-    monkeypatch.delenv("GEPI_ENGINE_GRAPH")
+    monkeypatch.setenv("GEPI_ENGINE_GRAPH", "1")
     with ge.Engine() as eng:
         for j, (panels, items) in enumerate(mixes):
             res, bs = run_batch(panels, items, eng)
