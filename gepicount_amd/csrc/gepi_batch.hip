@@ -251,7 +251,7 @@ int Engine::autotune(int32_t repeats) {
         int32_t n;
     };
     const Trial trials[4] = {{2, 16000, 256}, {8, 10000, 1024}, {32, 8000, 4096}, {INT32_MAX, 6000, 8192}};
-    const int per_cus[2] = {1, 2};
+    const int per_cus[3] = {0, 1, 2};
     const double bands[3] = {5e8, 2e9, 8e9};
     repeats = std::max(1, std::min(repeats, 16));
     std::vector<gepi_tune_class> table;
@@ -796,8 +796,8 @@ extern "C" int gepi_engine_set_tuning(gepi_engine *e, const gepi_tune_class *cla
         const gepi_tune_class &c = classes[i];
         if ((i > 0 && c.max_ksteps <= classes[i - 1].max_ksteps) || c.max_ksteps < 1)
             return gepi_fail(GEPI_ERR_ARG, "class %d: max_ksteps must increase", i);
-        if (c.wg_per_cu < 1 || c.wg_per_cu > 16)
-            return gepi_fail(GEPI_ERR_ARG, "class %d: wg_per_cu %d outside [1, 16]", i, c.wg_per_cu);
+        if (c.wg_per_cu < 0 || c.wg_per_cu > 16)
+            return gepi_fail(GEPI_ERR_ARG, "class %d: wg_per_cu %d outside [0, 16]", i, c.wg_per_cu);
         if (!(c.band_evals >= 1e6) || !std::isfinite(c.band_evals))
             return gepi_fail(GEPI_ERR_ARG, "class %d: band_evals below 1e6", i);
     }

@@ -230,8 +230,8 @@ def test_batch_graph_replays_equal_eager(monkeypatch):
 def test_batch_autotune_and_tables(monkeypatch):
     """gepi_engine_autotune returns one schedule per size class from its
     candidate set; batches under the tuned table and under a hand-set table
-    of extreme schedules (1 workgroup per CU and 64-tile bands for short
-    scans) equal the default engine's results and the oracle's; bad tables
+    of extreme schedules (1 workgroup per CU, one tile per workgroup,
+    64-tile bands) equal the default engine's results and the oracle's; bad tables
     are rejected and leave the engine usable."""
     print("case batch autotune", flush=True)
     monkeypatch.delenv("GEPI_F4_WG_PER_CU", raising=False)
@@ -242,15 +242,16 @@ def test_batch_autotune_and_tables(monkeypatch):
         print("autotune table", table, flush=True)
         assert [c["max_ksteps"] for c in table] == [2, 8, 32, 2**31 - 1]
         for c in table:
-            assert c["wg_per_cu"] in (1, 2) and c["band_evals"] in (5e8, 2e9, 8e9), c
+            assert c["wg_per_cu"] in (0, 1, 2) and c["band_evals"] in (5e8, 2e9, 8e9), c
             assert 0 < c["ms_best"] <= c["ms_default"], c
         tuned, _ = run_batch(panels, items, eng)
         eng.set_tuning([dict(max_ksteps=8, wg_per_cu=1, band_evals=1e6),
+                        dict(max_ksteps=20, wg_per_cu=0, band_evals=3e6),
                         dict(max_ksteps=2**31 - 1, wg_per_cu=2, band_evals=3e6)])
         extreme, bs2 = run_batch(panels, items, eng)
         assert bs2["rounds"] >= bs0["rounds"]
         for bad in ([dict(max_ksteps=8, wg_per_cu=2, band_evals=2e9)] * 2,
-                    [dict(max_ksteps=8, wg_per_cu=0, band_evals=2e9)],
+                    [dict(max_ksteps=8, wg_per_cu=-1, band_evals=2e9)],
                     [dict(max_ksteps=8, wg_per_cu=2, band_evals=10.0)]):
             with pytest.raises(ge.GepiError):
                 eng.set_tuning(bad)
