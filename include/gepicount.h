@@ -210,7 +210,8 @@ typedef struct gepi_tune_class {
  * (the default unless another is more than 3% faster).  The engine uses the
  * table from then on: a scan's band size by its class, a round's launch
  * form by the class with the most count work in it.  Results never depend
- * on the schedule.  Writes min(n_out, 4) classes; *n_classes = 4.          */
+ * on the schedule.  Writes min(n_out, 4) classes; *n_classes = 4.  On
+ * failure the engine is left with the default schedule.                    */
 int gepi_engine_autotune(gepi_engine *engine, int32_t repeats, gepi_tune_class *out,
                          int32_t n_out, int32_t *n_classes);
 /* Installs a table (from an earlier autotune): 1-16 classes, max_ksteps

@@ -162,3 +162,40 @@ def test_server_autotuned():
                    for r in reqs]
         got = [srv.wait(t) for t in tickets]
     check_against_alone_and_oracle(panels, reqs, got)
+
+
+def test_server_graph_rounds_with_concurrent_clients(monkeypatch):
+    """Captured-graph rounds (GEPI_ENGINE_GRAPH=1) in the server: the worker
+    thread captures and replays rounds while client threads upload panels
+    and submit; every response equals the oracle's."""
+    print("case server graph rounds", flush=True)
+    monkeypatch.setenv("GEPI_ENGINE_GRAPH", "1")
+    panels = make_panels(23, 6)
+    reqs = make_requests(24, panels, 24)
+    got = [None] * len(reqs)
+    with ge.Server() as srv:
+        ids = [None] * len(panels)
+        ids[0] = srv.add_panel_bed(panels[0]["payload"], panels[0]["n_var"], panels[0]["n"])
+
+        def client(j):
+            # each client uploads its own panels while other requests run
+            for pi in range(1 + j, len(panels), 3):
+                ids[pi] = srv.add_panel_bed(panels[pi]["payload"], panels[pi]["n_var"], panels[pi]["n"])
+            mine = [i for i in range(len(reqs)) if i % 3 == j]
+            tickets = []
+            for i in mine:
+                while ids[reqs[i]["pi"]] is None:
+                    time.sleep(0.001)
+                tickets.append((i, srv.submit(ids[reqs[i]["pi"]], reqs[i]["m0"], reqs[i]["m1"],
+                                              topmost=reqs[i]["k"], ffun=reqs[i]["ffun"])))
+            for i, t in tickets:
+                got[i] = srv.wait(t)
+
+        th = [threading.Thread(target=client, args=(j,)) for j in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == len(reqs)
+    check_against_alone_and_oracle(panels, reqs, got)
