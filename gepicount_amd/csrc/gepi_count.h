@@ -305,9 +305,9 @@ constexpr int F4_ROWS = F4_RA + TB4 * 3; /* plane rows of a tile: A, then B     
 constexpr int F4_UNITS = F4_ROWS * 4; /* uint4 of a staged super-step             */
 constexpr int F4_FMT = 4;       /* operand format code of e2m1                    */
 
-/* fp4 layout geometry: a super-step is 3 K steps of 128 samples; each class
+/* fp4 layout geometry: a super-step is 4 K steps of 128 samples; each class
  * starts on a super-step, class 0 at 0 and class 1 at f4_super(ks0).       */
-__host__ __device__ inline int32_t f4_super(int32_t ks) { return (ks + 2) / 3; }
+__host__ __device__ inline int32_t f4_super(int32_t ks) { return (ks + 3) / 4; }
 
 /* LDS layout of a staged super-step: the 288 rows of 4 uint4 unpadded, uint4
  * c of row r at unit 4r + (c ^ ((r >> 2) & 3)).  A fragment read (16 lanes:
@@ -319,7 +319,7 @@ __host__ __device__ inline int32_t f4_super(int32_t ks) { return (ks + 2) / 3; }
 __device__ __forceinline__ int f4_unit(int r, int c) { return 4 * r + (c ^ ((r >> 2) & 3)); }
 __device__ __forceinline__ int f4_src(int t) { return 4 * (t >> 2) + ((t & 3) ^ ((t >> 4) & 3)); }
 
-/* One lane's fp4 fragment of K step p of a super-step: the lane's four
+/* One lane's fp4 fragment of K step p < 3 of a super-step: the lane's four
  * layout dwords masked to nibble bit p.  A set bit is e2m1 0.5 * 2^p; with
  * block scale 2^(1-p) on both operands every product of two set bits is
  * exactly 1.0.  The order of samples within a K step does not change a dot
@@ -327,6 +327,65 @@ __device__ __forceinline__ int f4_src(int t) { return 4 * (t >> 2) + ((t & 3) ^ 
 __device__ __forceinline__ v8i frag_f4(const uint4 &w, uint32_t m) {
     return v8i{(int32_t)(w.x & m), (int32_t)(w.y & m), (int32_t)(w.z & m), (int32_t)(w.w & m),
                0, 0, 0, 0};
+}
+
+/* K step 3 is the e2m1 sign bit: its fragment keeps nibble bit 3 and sets
+ * bit 0, so every slot (padding included) is -0.5 when its sample bit is
+ * set and +0.5 when it is clear.  With block scale 1 a product is
+ * 0.25 (1 - 2a)(1 - 2b) = 1/4 - a/2 - b/2 + ab, so over the n3 slots of a
+ * class's fourth K steps the accumulator gains n3/4 - pa/2 - pb/2 + AND (pa,
+ * pb: the two rows' set bits there, from k_class_f4).  The accumulators
+ * start at pa/2 + pb/2 - n3/4 (f4_corr_init) and end at the plain count;
+ * every partial sum is a multiple of 1/4 far below 2^22, so exact.        */
+__device__ __forceinline__ v8i frag_f4s(const uint4 &w, uint32_t ones) {
+    return v8i{(int32_t)((w.x & 0x88888888u) | ones), (int32_t)((w.y & 0x88888888u) | ones),
+               (int32_t)((w.z & 0x88888888u) | ones), (int32_t)((w.w & 0x88888888u) | ones), 0, 0,
+               0, 0};
+}
+
+/* 0x11111111 in a vector register the optimizer cannot see through: with
+ * the mask in a scalar register, (w & mask) | ones is then one
+ * v_and_or_b32 (a three-operand instruction may read one scalar register
+ * and no literal; with both constants it becomes an and and an or).      */
+__device__ __forceinline__ uint32_t f4_ones_vgpr() {
+    uint32_t ones = 0x11111111u;
+    asm("" : "+v"(ones));
+    return ones;
+}
+
+/* Block scale (E8M0) of K step k: 2^(1-k) for k < 3, 1 for the sign step. */
+__device__ __forceinline__ constexpr int f4_scale(int k) { return k < 3 ? 128 - k : 127; }
+
+/* The fourth-K-step correction of a tile's 288 plane rows, per class: half
+ * the row's set bits there (LDS, written before the tile's first barrier). */
+struct F4Corr {
+    float h[2][F4_ROWS];
+};
+
+/* Global plane row of tile row r (A rows first, then B rows). */
+__device__ __forceinline__ int64_t f4_tile_row(int r, int32_t ta, int32_t tb) {
+    return r < F4_RA ? (int64_t)ta * F4_RA + r : (int64_t)tb * (TB4 * 3) + (r - F4_RA);
+}
+
+/* Starting accumulators of class c (ks: its K steps): element (i, p, q, r)
+ * is A row (32wa + 16i + 4lg + r)*3 + p against B row (16wb + l16)*3 + q. */
+__device__ __forceinline__ void f4_corr_init(v4f (&acc)[2][3][3], const F4Corr &cr, int c,
+                                             int32_t ks, int wa, int wb, int lg, int l16) {
+    const float n3q = 32.0f * (float)(ks / 4); /* n3 / 4, n3 = 128 per full super-step */
+    float hB[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) hB[q] = cr.h[c][F4_RA + (16 * wb + l16) * 3 + q] - n3q;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            float hA[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) hA[r] = cr.h[c][(32 * wa + 16 * i + 4 * lg + r) * 3 + p];
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                acc[i][p][q] = v4f{hA[0] + hB[q], hA[1] + hB[q], hA[2] + hB[q], hA[3] + hB[q]};
+        }
 }
 
 /* One 64 x 32-variant workgroup tile of the fp4 path (tile numbering as in
@@ -338,6 +397,7 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_UNITS];
     __shared__ PendStage stage;
     __shared__ PairList pairs;
+    __shared__ F4Corr corr;
     /* Fisher flags a variant pair in about 13% of lanes: those pairs are
      * parked and finished one per thread after the tile (entropy's few are
      * finished in the lane) */
@@ -386,6 +446,14 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
         if (hasB1) W[dB + D64] = pb1;
     };
     fetch(0);
+    /* the fourth-K-step corrections of the tile's rows (none without samples:
+     * the class layout was not built) */
+    const int64_t prow = (int64_t)g.n_var_pad * 3;
+    for (int r = tid; r < F4_ROWS; r += F4_THREADS) {
+        const int64_t gr = f4_tile_row(r, ta, tb);
+        corr.h[0][r] = g.n_ks > 0 ? 0.5f * (float)g.pc3[gr] : 0.0f;
+        corr.h[1][r] = g.n_ks > 0 ? 0.5f * (float)g.pc3[prow + gr] : 0.0f;
+    }
     put(lds[0]);
     fetch(last < 1 ? last : 1);
     __syncthreads();
@@ -398,6 +466,7 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
         oA[p] = f4_unit((32 * wa + l16) * 3 + p, lg);
         oB[p] = f4_unit(F4_RA + (16 * wb + l16) * 3 + p, lg);
     }
+    const uint32_t ones = f4_ones_vgpr();
     /* super-step u into one class's accumulators, its first np K steps
      * (the rest of a class's last super-step is padding); the staging
      * pipeline runs on across the two calls.                               */
@@ -412,21 +481,22 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 #pragma unroll
             for (int p = 0; p < 3; p++) wA[i][p] = L[oA[p] + 192 * i];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 4; k++) {
             if (k < np) {
                 const uint32_t m = 0x11111111u << k;
                 v8i b[3];
 #pragma unroll
-                for (int q = 0; q < 3; q++) b[q] = frag_f4(wB[q], m);
+                for (int q = 0; q < 3; q++) b[q] = k < 3 ? frag_f4(wB[q], m) : frag_f4s(wB[q], ones);
 #pragma unroll
                 for (int i = 0; i < 2; i++)
 #pragma unroll
                     for (int p = 0; p < 3; p++) {
-                        const v8i a = frag_f4(wA[i][p], m);
+                        const v8i a = k < 3 ? frag_f4(wA[i][p], m) : frag_f4s(wA[i][p], ones);
 #pragma unroll
                         for (int q = 0; q < 3; q++)
                             acc[i][p][q] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                                a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, 128 - k, 0, 128 - k);
+                                a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, f4_scale(k), 0,
+                                f4_scale(k));
                     }
             }
         }
@@ -440,25 +510,15 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
         __syncthreads();
     };
     v4f acc0[2][3][3], acc1[2][3][3];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-            for (int q = 0; q < 3; q++) acc0[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+    f4_corr_init(acc0, corr, 0, g.ks0, wa, wb, lg, l16);
     for (int32_t u = 0; u < u1; u++) {
-        const int32_t r = g.ks0 - 3 * u;
-        step(acc0, u, r < 3 ? r : 3);
+        const int32_t r = g.ks0 - 4 * u;
+        step(acc0, u, r < 4 ? r : 4);
     }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-            for (int q = 0; q < 3; q++) acc1[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+    f4_corr_init(acc1, corr, 1, ks1, wa, wb, lg, l16);
     for (int32_t u = u1; u < n_u; u++) {
-        const int32_t r = ks1 - 3 * (u - u1);
-        step(acc1, u, r < 3 ? r : 3);
+        const int32_t r = ks1 - 4 * (u - u1);
+        step(acc1, u, r < 4 ? r : 4);
     }
 
     /* epilogue: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
@@ -542,6 +602,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_UNITS];
     __shared__ PendStage stage;
     __shared__ F4Job s_job[2];
+    __shared__ F4Corr corr;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -600,6 +661,32 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         W[dB] = pb0;
         if (hasB1) W[dB + D64] = pb1;
     };
+    /* fourth-K-step corrections of a tile: this thread's rows tid and (tid <
+     * 32) tid + 256, both classes, fetched with the tile's first super-step
+     * and written to LDS with it (none without samples: the class layout
+     * was not built) */
+    uint32_t cr0 = 0u, cr1 = 0u; /* class 0 | class 1 << 16 (counts < 2^16) */
+    auto corr_fetch = [&](const CountArgs &g, const Geo &o) {
+        if (g.n_ks == 0) {
+            cr0 = cr1 = 0u;
+            return;
+        }
+        const int64_t prow = (int64_t)g.n_var_pad * 3;
+        const int64_t r0 = f4_tile_row(tid, o.ta, o.tb);
+        cr0 = (uint32_t)g.pc3[r0] | ((uint32_t)g.pc3[prow + r0] << 16);
+        if (tid < F4_ROWS - F4_THREADS) {
+            const int64_t r1 = f4_tile_row(tid + F4_THREADS, o.ta, o.tb);
+            cr1 = (uint32_t)g.pc3[r1] | ((uint32_t)g.pc3[prow + r1] << 16);
+        }
+    };
+    auto corr_put = [&]() {
+        corr.h[0][tid] = 0.5f * (float)(cr0 & 0xFFFFu);
+        corr.h[1][tid] = 0.5f * (float)(cr0 >> 16);
+        if (tid < F4_ROWS - F4_THREADS) {
+            corr.h[0][tid + F4_THREADS] = 0.5f * (float)(cr1 & 0xFFFFu);
+            corr.h[1][tid + F4_THREADS] = 0.5f * (float)(cr1 >> 16);
+        }
+    };
 
     if (tid == 0) {
         s_job[0] = src.grab(0);
@@ -611,12 +698,15 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     const CountArgs *gp = src.argp(job.item);
     Geo geo = geo_of(*gp, job.tile);
     fetch(geo, 0);
+    corr_fetch(*gp, geo);
     put(lds[0]);
+    corr_put();
     fetch(geo, 1);
     if (tid == 0) s_job[1] = src.grab(1);
     __syncthreads();
     if (geo.n_u == 0) __syncthreads();
 
+    const uint32_t ones = f4_ones_vgpr();
     for (int32_t t = 0;; t++) {
         const CountArgs &g = *gp;
         /* super-step u into one class's accumulators, its first np K steps
@@ -633,22 +723,22 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
 #pragma unroll
                 for (int p = 0; p < 3; p++) wA[i][p] = L[oA[p] + 192 * i];
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < 4; k++) {
                 if (k < np) {
                     const uint32_t m = 0x11111111u << k;
                     v8i b[3];
 #pragma unroll
-                    for (int q = 0; q < 3; q++) b[q] = frag_f4(wB[q], m);
+                    for (int q = 0; q < 3; q++) b[q] = k < 3 ? frag_f4(wB[q], m) : frag_f4s(wB[q], ones);
 #pragma unroll
                     for (int i = 0; i < 2; i++)
 #pragma unroll
                         for (int p = 0; p < 3; p++) {
-                            const v8i a = frag_f4(wA[i][p], m);
+                            const v8i a = k < 3 ? frag_f4(wA[i][p], m) : frag_f4s(wA[i][p], ones);
 #pragma unroll
                             for (int q = 0; q < 3; q++)
                                 acc[i][p][q] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                                    a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, 128 - k, 0,
-                                    128 - k);
+                                    a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, f4_scale(k), 0,
+                                    f4_scale(k));
                         }
                 }
             }
@@ -660,25 +750,15 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             __syncthreads();
         };
         v4f acc0[2][3][3], acc1[2][3][3];
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) acc0[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+        f4_corr_init(acc0, corr, 0, g.ks0, wa, wb, lg, l16);
         for (int32_t u = 0; u < geo.u1; u++) {
-            const int32_t r = g.ks0 - 3 * u;
-            step(acc0, u, r < 3 ? r : 3);
+            const int32_t r = g.ks0 - 4 * u;
+            step(acc0, u, r < 4 ? r : 4);
         }
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) acc1[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
+        f4_corr_init(acc1, corr, 1, geo.ks1, wa, wb, lg, l16);
         for (int32_t u = geo.u1; u < geo.n_u; u++) {
-            const int32_t r = geo.ks1 - 3 * (u - geo.u1);
-            step(acc1, u, r < 3 ? r : 3);
+            const int32_t r = geo.ks1 - 4 * (u - geo.u1);
+            step(acc1, u, r < 4 ? r : 4);
         }
 
         /* the filter's constants are read before the next tile's loads are
@@ -693,6 +773,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             gpn = src.argp(nj.item);
             geon = geo_of(*gpn, nj.tile);
             fetch(geon, 0);
+            corr_fetch(*gpn, geon);
         }
 
         /* filter: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
@@ -736,6 +817,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         const bool flush = nj.item != job.item || stage.n >= (uint32_t)(PEND_LDS / 2);
         if (flush) stage_flush_after_barrier(g, stage);
         put(lds[0]);
+        corr_put(); /* this tile's corrections were last read before the barrier above */
         fetch(geon, 1);
         __syncthreads(); /* super-step 0 staged; the flush's copies are done */
         if (tid == 0) {

@@ -293,13 +293,18 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     if (path != 3 &&
         (rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK)
         return rc;
-    /* int8: n_ks 64-sample words per plane row; fp4: n_u 64-byte super-step rows */
+    /* int8: n_ks 64-sample words per plane row; fp4: n_u 64-byte super-step
+     * rows, then the rows' fourth-K-step set bits ([2][n_var_pad * 3] int32) */
     const int32_t n_u = f4_super(w0 / 4) + f4_super(w1 / 4);
+    const size_t f4_lay = (size_t)std::max(n_u, 1) * n_var_pad * 3 * 64;
     if (path >= 2 &&
         (rc = grow(&p->d_clsk, &p->clsk_bytes,
-                   path == 3 ? (size_t)std::max(n_u, 1) * n_var_pad * 3 * 64
+                   path == 3 ? f4_lay + (size_t)2 * n_var_pad * 3 * sizeof(int32_t)
                              : (size_t)std::max(n_ks, 1) * n_var_pad * 3 * 8)) != GEPI_OK)
         return rc;
+    int32_t *d_pc3 = path == 3 ? reinterpret_cast<int32_t *>(
+                                     reinterpret_cast<char *>(p->d_clsk) + f4_lay)
+                               : nullptr;
     if ((rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK) return rc;
     /* fp4 path: per 32-sample word of the panel, the compression of each
      * class (mask bits at or beyond n_samples are dropped) */
@@ -357,7 +362,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         if (n_ks > 0) {
             k_class_f4<<<(unsigned)((rows + 255) / 256), 256, 0, p->stream>>>(
                 p->d_nat, (int32_t)n_var, p->w64, p->d_cw, rows, f4_super(w0 / 4),
-                reinterpret_cast<uint4 *>(p->d_clsk));
+                reinterpret_cast<uint4 *>(p->d_clsk), d_pc3);
             HIP_TRY(hipGetLastError());
         }
     } else {
@@ -392,6 +397,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     memset(&ca, 0, sizeof ca);
     ca.planes = p->d_cls;
     ca.planes_k = p->d_clsk;
+    ca.pc3 = d_pc3;
     ca.n_var = (int32_t)n_var;
     ca.n_var_pad = (int32_t)n_var_pad;
     ca.wc = wc;

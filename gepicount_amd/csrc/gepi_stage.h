@@ -18,9 +18,10 @@
  * k_class_f4       panel -> fp4 MFMA layout directly: per plane row, the
  *                  samples of each class are compressed (stable) out of the
  *                  file order with mask-driven bit compression, cut into
- *                  128-sample K steps and written three K steps at a time as
+ *                  128-sample K steps and written four K steps at a time as
  *                  64-byte super-step rows (class 0, then class 1 from the
- *                  next super-step).
+ *                  next super-step), with each row's set bits in the fourth
+ *                  K steps.
  * The QA kernels replace the loops of checkQuality (src/reader/
  * PlinkReader.h:237-335), which count missing calls on the host.
  */
@@ -160,55 +161,61 @@ __device__ __forceinline__ uint32_t spread8(uint32_t x) {
     return x;
 }
 
-/* One 64-byte super-step row of the fp4 layout from the row's three
- * 128-sample K steps k0, k1, k2 (32-bit word g of a K step holds its samples
+/* One 64-byte super-step row of the fp4 layout from the row's four
+ * 128-sample K steps k0..k3 (32-bit word g of a K step holds its samples
  * 32g .. 32g+31): dword d of 16-byte group g holds, at bit 4n + p, sample
- * 32g + 8d + n of K step p.  Bit 3 of every nibble stays zero.            */
-__device__ __forceinline__ uint4 super_group(uint32_t a, uint32_t b, uint32_t c) {
+ * 32g + 8d + n of K step p (bit 3, the e2m1 sign bit, for K step 3).      */
+__device__ __forceinline__ uint4 super_group(uint32_t a, uint32_t b, uint32_t c, uint32_t e) {
     uint4 o;
-    o.x = spread8(a) | (spread8(b) << 1) | (spread8(c) << 2);
-    o.y = spread8(a >> 8) | (spread8(b >> 8) << 1) | (spread8(c >> 8) << 2);
-    o.z = spread8(a >> 16) | (spread8(b >> 16) << 1) | (spread8(c >> 16) << 2);
-    o.w = spread8(a >> 24) | (spread8(b >> 24) << 1) | (spread8(c >> 24) << 2);
+    o.x = spread8(a) | (spread8(b) << 1) | (spread8(c) << 2) | (spread8(e) << 3);
+    o.y = spread8(a >> 8) | (spread8(b >> 8) << 1) | (spread8(c >> 8) << 2) | (spread8(e >> 8) << 3);
+    o.z = spread8(a >> 16) | (spread8(b >> 16) << 1) | (spread8(c >> 16) << 2) |
+          (spread8(e >> 16) << 3);
+    o.w = spread8(a >> 24) | (spread8(b >> 24) << 1) | (spread8(c >> 24) << 2) |
+          (spread8(e >> 24) << 3);
     return o;
 }
 
 /* A stream of bits appended in order, cut into 128-sample K steps, and
- * written three K steps at a time as super-step rows
- * out[(u * rows + r) * 4 + g].  Every thread of a launch appends the same
- * counts in the same order, so the bookkeeping is uniform.  The completed
- * K steps of the current super-step advance through k0, k1 (an indexed
- * array would be placed in scratch memory).                                */
+ * written four K steps at a time as super-step rows
+ * out[(u * rows + r) * 4 + g]; pc3 counts the set bits of every fourth K
+ * step (the count kernels' correction of that step).  Every thread of a
+ * launch appends the same counts in the same order, so the bookkeeping is
+ * uniform.  The completed K steps of the current super-step advance
+ * through k0, k1, k2 (an indexed array would be placed in scratch memory). */
 struct SuperStream {
     uint64_t acc;
-    int32_t na, nq, np;
+    int32_t na, nq, np, pc3;
     int64_t u;
     uint32_t q0, q1, q2, q3;
-    uint4 k0, k1;
+    uint4 k0, k1, k2;
 
     __device__ __forceinline__ void init(int64_t u0) {
         acc = 0;
-        na = nq = np = 0;
+        na = nq = np = pc3 = 0;
         u = u0;
         q0 = q1 = q2 = q3 = 0;
-        k0 = k1 = make_uint4(0u, 0u, 0u, 0u);
+        k0 = k1 = k2 = make_uint4(0u, 0u, 0u, 0u);
     }
     __device__ __forceinline__ void kstep(const uint4 &kr, uint4 *out, int64_t rows, int64_t r) {
         if (np == 0) {
             k0 = kr;
         } else if (np == 1) {
             k1 = kr;
+        } else if (np == 2) {
+            k2 = kr;
         } else {
             if (r < rows) {
                 uint4 *dst = out + (u * rows + r) * 4;
-                dst[0] = super_group(k0.x, k1.x, kr.x);
-                dst[1] = super_group(k0.y, k1.y, kr.y);
-                dst[2] = super_group(k0.z, k1.z, kr.z);
-                dst[3] = super_group(k0.w, k1.w, kr.w);
+                dst[0] = super_group(k0.x, k1.x, k2.x, kr.x);
+                dst[1] = super_group(k0.y, k1.y, k2.y, kr.y);
+                dst[2] = super_group(k0.z, k1.z, k2.z, kr.z);
+                dst[3] = super_group(k0.w, k1.w, k2.w, kr.w);
             }
+            pc3 += __popc(kr.x) + __popc(kr.y) + __popc(kr.z) + __popc(kr.w);
             u++;
         }
-        np = np == 2 ? 0 : np + 1;
+        np = np == 3 ? 0 : np + 1;
     }
     __device__ __forceinline__ void emit(uint32_t w, uint4 *out, int64_t rows, int64_t r) {
         q0 = q1;
@@ -245,14 +252,16 @@ struct SuperStream {
 /* One thread per plane row r = v*3 + p of [0, rows) (rows = n_var_pad * 3;
  * rows of padding variants v >= n_var are zero).  cw: [2][2*w64]
  * CompressWord, class 0 then class 1.  Class 0 fills super-steps [0, u1)
- * (u1 = f4_super(its K steps)), class 1 the super-steps from u1 on.  The
- * panel rows of a workgroup are staged through LDS 8 words at a time
- * (coalesced reads).                                                       */
+ * (u1 = f4_super(its K steps)), class 1 the super-steps from u1 on.
+ * pc3[c * rows + r]: set bits of row r in the fourth K steps of class c's
+ * super-steps.  The panel rows of a workgroup are staged through LDS 8
+ * words at a time (coalesced reads).                                       */
 constexpr int CLS_CHUNK = 8;
 static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
                                                    int32_t w64, const CompressWord *__restrict__ cw,
                                                    int64_t rows, int32_t u1,
-                                                   uint4 *__restrict__ out) {
+                                                   uint4 *__restrict__ out,
+                                                   int32_t *__restrict__ pc3) {
     __shared__ uint64_t srow[256][CLS_CHUNK + 1];
     const int64_t r0 = (int64_t)blockIdx.x * 256;
     const int64_t r = r0 + threadIdx.x;
@@ -286,6 +295,10 @@ static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restr
     }
     s0.finish(out, rows, r);
     s1.finish(out, rows, r);
+    if (r < rows) {
+        pc3[r] = s0.pc3;
+        pc3[rows + r] = s1.pc3;
+    }
 }
 
 /* missing[v] = n_samples - popc(valid plane of variant v); 64 threads per

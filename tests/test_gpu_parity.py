@@ -127,6 +127,30 @@ def test_fp4_filter_count_types(n0, n1, ffun):
     assert_same(got, ref, "fp4-filter")
 
 
+@pytest.mark.parametrize("n0,n1", [(512, 1024), (640, 384), (768, 1), (1, 896), (1024, 0),
+                                   (500, 1020), (130, 4), (3, 2)])
+def test_fp4_sign_step_residues(n0, n1):
+    """A super-step of the fp4 layout holds 4 K steps of 128 samples, the
+    fourth in the e2m1 sign bit with a per-row correction; each class
+    starts a new super-step.  Class sizes cover 0, 1, 2 and 3 K steps past
+    full super-steps in either class, exact multiples of 512 samples, one
+    or no sample in a class, and partial last K steps.  Every evaluation
+    must equal the oracle's."""
+    print("case fp4 sign step n0=%d n1=%d" % (n0, n1), flush=True)
+    rng = np.random.default_rng(91 + n0 + 7 * n1)
+    n = n0 + n1
+    codes = orc.random_codes(rng, 21, n, 0.03)
+    cls = rng.permutation(np.r_[np.zeros(n0, np.int64), np.ones(n1, np.int64)])
+    bits, valid = orc.planes_from_codes(codes)
+    m0, m1 = orc.masks_from_classes(cls)
+    R = bits.shape[0]
+    k = 5 * R * (R + 1) // 2
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k, count_path=3)
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), k)
+    assert st["n_class0"] == n0 and st["n_class1"] == n1
+    assert_same(got, ref, "fp4 sign step")
+
+
 def test_fisher_parked_pairs_overflow(monkeypatch):
     """The fp4 single-scan kernel parks variant pairs that pass the
     chi-square bound in LDS and finishes them (function bits, Fisher's exact
