@@ -347,14 +347,13 @@ size_t Engine::estimate_footprint(const gepi_panel *p, int64_t topmost) {
     const int64_t n_var_pad = ((p->n_var + TM - 1) / TM) * TM;
     /* super-steps of a scan whose classes hold every sample (a sample in
      * both masks, counted twice, is not assumed) */
-    const int64_t n_u = ((p->n_samples + 127) / 128 + 3) / 4 + 2;
+    const int64_t n_u = ((p->n_samples + 127) / 128 + 2) / 3 + 2;
     const int64_t R = 2 * p->n_var;
     const int64_t k = std::min<int64_t>(topmost, 5 * (R * (R + 1) / 2));
     const unsigned long long cap =
         std::max<unsigned long long>((unsigned long long)k + TILE_EVALS_MAX + 1024, 4ull * k);
     const unsigned long long pcap = std::max<unsigned long long>(cap, 1ull << 20);
-    return round_block((size_t)n_u * n_var_pad * 3 * 64 + (size_t)2 * n_var_pad * 3 * sizeof(int32_t)) +
-           round_block(cap * sizeof(Cand)) +
+    return round_block((size_t)n_u * n_var_pad * 3 * 64) + round_block(cap * sizeof(Cand)) +
            round_block(pcap * sizeof(Pend)) + round_block((size_t)4 * p->w64 * sizeof(CompressWord)) +
            round_block(sizeof(ScanState));
 }
@@ -366,7 +365,6 @@ void Engine::drop(ScanItem *it, int status) {
     dfree(it->d_cand);
     dfree(it->d_state);
     it->d_layout = nullptr;
-    it->d_pc3 = nullptr;
     it->d_cw = nullptr;
     it->d_pend = nullptr;
     it->d_cand = nullptr;
@@ -444,14 +442,10 @@ int Engine::add(ScanItem *it) {
     if (getenv("GEPI_PEND_CAP"))
         it->pcap = std::max<unsigned long long>(env_ull("GEPI_PEND_CAP", 0),
                                                 (unsigned long long)TILE_EVALS_MAX + 1024);
-    const int32_t n_u = f4_super(w0 / 4) + f4_super(w1 / 4); /* 64-byte rows each */
-    /* super-step rows, then the rows' fourth-K-step set bits ([2][rows] int32) */
-    const size_t lay0 = (size_t)std::max(n_u, 1) * it->n_var_pad * 3 * 4 * sizeof(uint4);
-    const size_t lay = lay0 + (size_t)2 * it->n_var_pad * 3 * sizeof(int32_t);
+    /* three K steps per super-step (f4_super3, see count_f4_run) */
+    const int32_t n_u = f4_super3(w0 / 4) + f4_super3(w1 / 4); /* 64-byte rows each */
+    const size_t lay = (size_t)std::max(n_u, 1) * it->n_var_pad * 3 * 4 * sizeof(uint4);
     it->d_layout = (uint4 *)dalloc(lay);
-    it->d_pc3 = it->d_layout
-                    ? reinterpret_cast<int32_t *>(reinterpret_cast<char *>(it->d_layout) + lay0)
-                    : nullptr;
     it->d_cw = (CompressWord *)dalloc(it->h_cw.size() * sizeof(CompressWord));
     it->d_pend = (Pend *)dalloc(it->pcap * sizeof(Pend));
     it->d_cand = (Cand *)dalloc(it->cap * sizeof(Cand));
@@ -473,9 +467,9 @@ int Engine::add(ScanItem *it) {
                            stream));
     if (it->n_k4 > 0) {
         const int64_t rows = it->n_var_pad * 3;
-        k_class_f4<<<(unsigned)((rows + 255) / 256), 256, 0, stream>>>(
-            p->d_nat, (int32_t)it->n_var, p->w64, it->d_cw, rows, f4_super(it->ks0),
-            it->d_layout, it->d_pc3);
+        k_class_f4<3><<<(unsigned)((rows + 255) / 256), 256, 0, stream>>>(
+            p->d_nat, (int32_t)it->n_var, p->w64, it->d_cw, rows, f4_super3(it->ks0),
+            it->d_layout, nullptr);
         HIP_TRY(hipGetLastError());
     }
     const gepi_tune_class *tc = tune_of(it->n_k4);
@@ -588,7 +582,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             BatchDesc d;
             memset(&d, 0, sizeof d);
             d.g.planes_k = reinterpret_cast<const uint64_t *>(it->d_layout);
-            d.g.pc3 = it->d_pc3;
+            d.g.pc3 = nullptr; /* three-K-step layout: no correction */
             d.g.tile_begin = it->t0;
             d.g.n_tiles = (int32_t)it->band_used;
             d.g.n_var = (int32_t)it->n_var;

@@ -305,9 +305,12 @@ constexpr int F4_ROWS = F4_RA + TB4 * 3; /* plane rows of a tile: A, then B     
 constexpr int F4_UNITS = F4_ROWS * 4; /* uint4 of a staged super-step             */
 constexpr int F4_FMT = 4;       /* operand format code of e2m1                    */
 
-/* fp4 layout geometry: a super-step is 4 K steps of 128 samples; each class
- * starts on a super-step, class 0 at 0 and class 1 at f4_super(ks0).       */
+/* fp4 layout geometry: a super-step is 4 K steps of 128 samples (single
+ * scans; the batch engine's layout has 3, f4_super3, see count_f4_run);
+ * each class starts on a super-step, class 0 at 0 and class 1 at
+ * f4_super(ks0).                                                          */
 __host__ __device__ inline int32_t f4_super(int32_t ks) { return (ks + 3) / 4; }
+__host__ __device__ inline int32_t f4_super3(int32_t ks) { return (ks + 2) / 3; }
 
 /* LDS layout of a staged super-step: the 288 rows of 4 uint4 unpadded, uint4
  * c of row r at unit 4r + (c ^ ((r >> 2) & 3)).  A fragment read (16 lanes:
@@ -562,7 +565,11 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 }
 
 /* Persistent fp4 count loop (batched scans, whose tiles are short and come
- * from many scans).  A launch has a few workgroups per CU; each takes tiles
+ * from many scans).  Its layout has three K steps per super-step (nibble
+ * bits 0-2, f4_super3): the sign-bit fourth step, which needs the
+ * per-row correction, measured 11% slower on batches of Fisher scans
+ * (larger code in this already large kernel) and no faster on entropy
+ * batches.  A launch has a few workgroups per CU; each takes tiles
  * from the work counter of its XCD (workgroups are
  * dispatched to the 8 XCDs round robin, XCD = blockIdx.x % 8) until that
  * XCD's share of the band is done.  Per tile: the super-step pipeline into
@@ -602,7 +609,6 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_UNITS];
     __shared__ PendStage stage;
     __shared__ F4Job s_job[2];
-    __shared__ F4Corr corr;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -640,8 +646,8 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         o.gA = base + (int64_t)o.ta * (F4_RA * 4) + si;
         o.gB = base + (int64_t)o.tb * (TB4 * 3 * 4) + si;
         o.ks1 = g.n_ks - g.ks0;
-        o.u1 = f4_super(g.ks0);
-        o.n_u = o.u1 + f4_super(o.ks1);
+        o.u1 = f4_super3(g.ks0);
+        o.n_u = o.u1 + f4_super3(o.ks1);
         o.last = o.n_u > 0 ? o.n_u - 1 : 0; /* n_u == 0: no K loop, super-step 0 exists */
         return o;
     };
@@ -661,32 +667,6 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         W[dB] = pb0;
         if (hasB1) W[dB + D64] = pb1;
     };
-    /* fourth-K-step corrections of a tile: this thread's rows tid and (tid <
-     * 32) tid + 256, both classes, fetched with the tile's first super-step
-     * and written to LDS with it (none without samples: the class layout
-     * was not built) */
-    uint32_t cr0 = 0u, cr1 = 0u; /* class 0 | class 1 << 16 (counts < 2^16) */
-    auto corr_fetch = [&](const CountArgs &g, const Geo &o) {
-        if (g.n_ks == 0) {
-            cr0 = cr1 = 0u;
-            return;
-        }
-        const int64_t prow = (int64_t)g.n_var_pad * 3;
-        const int64_t r0 = f4_tile_row(tid, o.ta, o.tb);
-        cr0 = (uint32_t)g.pc3[r0] | ((uint32_t)g.pc3[prow + r0] << 16);
-        if (tid < F4_ROWS - F4_THREADS) {
-            const int64_t r1 = f4_tile_row(tid + F4_THREADS, o.ta, o.tb);
-            cr1 = (uint32_t)g.pc3[r1] | ((uint32_t)g.pc3[prow + r1] << 16);
-        }
-    };
-    auto corr_put = [&]() {
-        corr.h[0][tid] = 0.5f * (float)(cr0 & 0xFFFFu);
-        corr.h[1][tid] = 0.5f * (float)(cr0 >> 16);
-        if (tid < F4_ROWS - F4_THREADS) {
-            corr.h[0][tid + F4_THREADS] = 0.5f * (float)(cr1 & 0xFFFFu);
-            corr.h[1][tid + F4_THREADS] = 0.5f * (float)(cr1 >> 16);
-        }
-    };
 
     if (tid == 0) {
         s_job[0] = src.grab(0);
@@ -698,15 +678,12 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     const CountArgs *gp = src.argp(job.item);
     Geo geo = geo_of(*gp, job.tile);
     fetch(geo, 0);
-    corr_fetch(*gp, geo);
     put(lds[0]);
-    corr_put();
     fetch(geo, 1);
     if (tid == 0) s_job[1] = src.grab(1);
     __syncthreads();
     if (geo.n_u == 0) __syncthreads();
 
-    const uint32_t ones = f4_ones_vgpr();
     for (int32_t t = 0;; t++) {
         const CountArgs &g = *gp;
         /* super-step u into one class's accumulators, its first np K steps
@@ -723,22 +700,22 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
 #pragma unroll
                 for (int p = 0; p < 3; p++) wA[i][p] = L[oA[p] + 192 * i];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < 3; k++) {
                 if (k < np) {
                     const uint32_t m = 0x11111111u << k;
                     v8i b[3];
 #pragma unroll
-                    for (int q = 0; q < 3; q++) b[q] = k < 3 ? frag_f4(wB[q], m) : frag_f4s(wB[q], ones);
+                    for (int q = 0; q < 3; q++) b[q] = frag_f4(wB[q], m);
 #pragma unroll
                     for (int i = 0; i < 2; i++)
 #pragma unroll
                         for (int p = 0; p < 3; p++) {
-                            const v8i a = k < 3 ? frag_f4(wA[i][p], m) : frag_f4s(wA[i][p], ones);
+                            const v8i a = frag_f4(wA[i][p], m);
 #pragma unroll
                             for (int q = 0; q < 3; q++)
                                 acc[i][p][q] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                                    a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, f4_scale(k), 0,
-                                    f4_scale(k));
+                                    a, b[q], acc[i][p][q], F4_FMT, F4_FMT, 0, 128 - k, 0,
+                                    128 - k);
                         }
                 }
             }
@@ -750,15 +727,25 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             __syncthreads();
         };
         v4f acc0[2][3][3], acc1[2][3][3];
-        f4_corr_init(acc0, corr, 0, g.ks0, wa, wb, lg, l16);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc0[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
         for (int32_t u = 0; u < geo.u1; u++) {
-            const int32_t r = g.ks0 - 4 * u;
-            step(acc0, u, r < 4 ? r : 4);
+            const int32_t r = g.ks0 - 3 * u;
+            step(acc0, u, r < 3 ? r : 3);
         }
-        f4_corr_init(acc1, corr, 1, geo.ks1, wa, wb, lg, l16);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc1[i][p][q] = v4f{0.f, 0.f, 0.f, 0.f};
         for (int32_t u = geo.u1; u < geo.n_u; u++) {
-            const int32_t r = geo.ks1 - 4 * (u - geo.u1);
-            step(acc1, u, r < 4 ? r : 4);
+            const int32_t r = geo.ks1 - 3 * (u - geo.u1);
+            step(acc1, u, r < 3 ? r : 3);
         }
 
         /* the filter's constants are read before the next tile's loads are
@@ -773,7 +760,6 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             gpn = src.argp(nj.item);
             geon = geo_of(*gpn, nj.tile);
             fetch(geon, 0);
-            corr_fetch(*gpn, geon);
         }
 
         /* filter: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
@@ -817,7 +803,6 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         const bool flush = nj.item != job.item || stage.n >= (uint32_t)(PEND_LDS / 2);
         if (flush) stage_flush_after_barrier(g, stage);
         put(lds[0]);
-        corr_put(); /* this tile's corrections were last read before the barrier above */
         fetch(geon, 1);
         __syncthreads(); /* super-step 0 staged; the flush's copies are done */
         if (tid == 0) {

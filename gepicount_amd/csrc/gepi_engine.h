@@ -49,7 +49,6 @@ struct ScanItem {
     int32_t n0 = 0, n1 = 0, n_k4 = 0, ks0 = 0;
     unsigned long long cap = 0, pcap = 0, count = 0, exact_total = 0;
     uint4 *d_layout = nullptr;
-    int32_t *d_pc3 = nullptr; /* inside d_layout: fourth-K-step set bits per row */
     gepi::CompressWord *d_cw = nullptr;
     gepi::Pend *d_pend = nullptr;
     gepi::Cand *d_cand = nullptr;

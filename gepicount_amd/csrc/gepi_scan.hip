@@ -360,7 +360,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     if (path == 3) {
         const int64_t rows = n_var_pad * 3;
         if (n_ks > 0) {
-            k_class_f4<<<(unsigned)((rows + 255) / 256), 256, 0, p->stream>>>(
+            k_class_f4<4><<<(unsigned)((rows + 255) / 256), 256, 0, p->stream>>>(
                 p->d_nat, (int32_t)n_var, p->w64, p->d_cw, rows, f4_super(w0 / 4),
                 reinterpret_cast<uint4 *>(p->d_clsk), d_pc3);
             HIP_TRY(hipGetLastError());

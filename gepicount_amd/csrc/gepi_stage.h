@@ -177,12 +177,13 @@ __device__ __forceinline__ uint4 super_group(uint32_t a, uint32_t b, uint32_t c,
 }
 
 /* A stream of bits appended in order, cut into 128-sample K steps, and
- * written four K steps at a time as super-step rows
- * out[(u * rows + r) * 4 + g]; pc3 counts the set bits of every fourth K
- * step (the count kernels' correction of that step).  Every thread of a
- * launch appends the same counts in the same order, so the bookkeeping is
- * uniform.  The completed K steps of the current super-step advance
+ * written S (3 or 4) K steps at a time as super-step rows
+ * out[(u * rows + r) * 4 + g]; with S = 4, pc3 counts the set bits of every
+ * fourth K step (the count kernel's correction of that step).  Every thread
+ * of a launch appends the same counts in the same order, so the bookkeeping
+ * is uniform.  The completed K steps of the current super-step advance
  * through k0, k1, k2 (an indexed array would be placed in scratch memory). */
+template <int S>
 struct SuperStream {
     uint64_t acc;
     int32_t na, nq, np, pc3;
@@ -202,20 +203,22 @@ struct SuperStream {
             k0 = kr;
         } else if (np == 1) {
             k1 = kr;
-        } else if (np == 2) {
+        } else if (S == 4 && np == 2) {
             k2 = kr;
         } else {
+            /* S = 3: kr is the third K step, bit 3 of every nibble stays 0 */
+            const uint4 c = S == 4 ? k2 : kr, e = S == 4 ? kr : make_uint4(0u, 0u, 0u, 0u);
             if (r < rows) {
                 uint4 *dst = out + (u * rows + r) * 4;
-                dst[0] = super_group(k0.x, k1.x, k2.x, kr.x);
-                dst[1] = super_group(k0.y, k1.y, k2.y, kr.y);
-                dst[2] = super_group(k0.z, k1.z, k2.z, kr.z);
-                dst[3] = super_group(k0.w, k1.w, k2.w, kr.w);
+                dst[0] = super_group(k0.x, k1.x, c.x, e.x);
+                dst[1] = super_group(k0.y, k1.y, c.y, e.y);
+                dst[2] = super_group(k0.z, k1.z, c.z, e.z);
+                dst[3] = super_group(k0.w, k1.w, c.w, e.w);
             }
-            pc3 += __popc(kr.x) + __popc(kr.y) + __popc(kr.z) + __popc(kr.w);
+            pc3 += __popc(e.x) + __popc(e.y) + __popc(e.z) + __popc(e.w);
             u++;
         }
-        np = np == 3 ? 0 : np + 1;
+        np = np == S - 1 ? 0 : np + 1;
     }
     __device__ __forceinline__ void emit(uint32_t w, uint4 *out, int64_t rows, int64_t r) {
         q0 = q1;
@@ -251,13 +254,15 @@ struct SuperStream {
 
 /* One thread per plane row r = v*3 + p of [0, rows) (rows = n_var_pad * 3;
  * rows of padding variants v >= n_var are zero).  cw: [2][2*w64]
- * CompressWord, class 0 then class 1.  Class 0 fills super-steps [0, u1)
- * (u1 = f4_super(its K steps)), class 1 the super-steps from u1 on.
- * pc3[c * rows + r]: set bits of row r in the fourth K steps of class c's
- * super-steps.  The panel rows of a workgroup are staged through LDS 8
- * words at a time (coalesced reads).                                       */
+ * CompressWord, class 0 then class 1.  S K steps per super-step (4: single
+ * scans, 3: the batch engine).  Class 0 fills super-steps [0, u1) (u1 =
+ * f4_super or f4_super3 of its K steps), class 1 the super-steps from u1
+ * on.  S = 4: pc3[c * rows + r] = set bits of row r in the fourth K steps
+ * of class c's super-steps.  The panel rows of a workgroup are staged
+ * through LDS 8 words at a time (coalesced reads).                         */
 constexpr int CLS_CHUNK = 8;
-static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
+template <int S>
+__global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
                                                    int32_t w64, const CompressWord *__restrict__ cw,
                                                    int64_t rows, int32_t u1,
                                                    uint4 *__restrict__ out,
@@ -266,7 +271,7 @@ static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restr
     const int64_t r0 = (int64_t)blockIdx.x * 256;
     const int64_t r = r0 + threadIdx.x;
     const int64_t real_rows = (int64_t)n_var * 3;
-    SuperStream s0, s1;
+    SuperStream<S> s0, s1;
     s0.init(0);
     s1.init(u1);
     const CompressWord *c0 = cw, *c1 = cw + 2 * w64;
@@ -295,7 +300,7 @@ static __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restr
     }
     s0.finish(out, rows, r);
     s1.finish(out, rows, r);
-    if (r < rows) {
+    if (S == 4 && r < rows) {
         pc3[r] = s0.pc3;
         pc3[rows + r] = s1.pc3;
     }
