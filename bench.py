@@ -315,8 +315,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine runs on one GPU per process and "
+              "has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "mix64":
         run_mix(args)
