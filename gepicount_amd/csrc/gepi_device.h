@@ -52,9 +52,12 @@ struct CountArgs {
     const uint32_t *planes;   /* popcount layout [n_var_pad][3][wc] dwords          */
     const uint64_t *planes_k; /* int8: [n_ks][n_var_pad][3] 64-sample words;
                                * fp4: [n_u][n_var_pad*3][4] uint4 super-step rows
-                               * (4 K steps of 128 samples; see k_class_f4)        */
-    const int32_t *pc3;       /* fp4: [2][n_var_pad*3] set bits of each plane row
-                               * in its class's fourth K steps (k_class_f4)        */
+                               * (K steps of 128 samples: 4 per super-step for
+                               * single scans, 3 in the batch engine; see
+                               * k_class_f4)                                      */
+    const int32_t *pc3;       /* fp4 single scans: [2][n_var_pad*3] set bits of
+                               * each plane row in its class's fourth K steps
+                               * (k_class_f4); unused by the batch engine         */
     int64_t tile_begin;       /* first linear tile of the band                      */
     int32_t n_tiles;          /* tiles in the band (= grid size)                    */
     int32_t n_var;
