@@ -110,7 +110,7 @@ __host__ __device__ inline void tile_of_f4(int64_t t, int32_t n_a, int32_t &ta, 
     const int64_t l = t - a0 * (a0 + 1);
     int64_t ra, rb;
     if (l < P * W) {
-        rb = P == F4_PANEL ? l >> 3 : l / P;
+        rb = P == F4_PANEL ? l / F4_PANEL : l / P; /* a shift for a full panel */
         ra = a0 + (l - rb * P);
     } else {
         const int64_t e = l - P * W; /* i(i-1) tiles of the rest precede row a0+i */
