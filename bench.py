@@ -39,9 +39,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="large",
-                    choices=["small", "medium", "large", "mix64", "serve"],
+                    choices=["small", "medium", "large", "mix64", "serve", "cpuref"],
                     help="mix64: 64 uneven scans, batched (gepi_engine) against sequential; "
-                         "serve: Poisson arrivals on the serving loop (gepi_server)")
+                         "serve: Poisson arrivals on the serving loop (gepi_server); "
+                         "cpuref: the reference-style one-thread CPU gold on the small "
+                         "workload and the medium workload's first 2,000 variants")
     ap.add_argument("--qps", type=float, default=40.0, help="serve: arrival rate")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the run")
     ap.add_argument("--n-var", type=int, default=None, help="override variant count")
@@ -95,9 +97,35 @@ def cpu_baseline(panel, m0, m1, n_samples, target_s, topk, ffun):
     v2 = int(v * (target_s / max(dt, 1e-3)) ** 0.5)
     v2 = max(v, min(v2, bits.shape[0] // 2))
     ev, dt = run(v2)
-    return {"value": ev / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "oracle gold scan (OpenMP, exact scores, top-%d) of the first %d variants "
-                      "(%d evals) of the same panel, %.1f s" % (topk, v2, ev, dt)}
+    out = {"value": ev / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+           "sample": "oracle gold scan (OpenMP, exact scores, top-%d) of the first %d variants "
+                     "(%d evals) of the same panel, %.1f s" % (topk, v2, ev, dt)}
+    if ffun == 0:
+        out["reference_style"] = reference_style(orc, bits, valid, m0, m1, n_samples, topk,
+                                                 target_s / 3.0)
+    return out
+
+
+def reference_style(orc, bits, valid, m0, m1, n_samples, topk, target_s):
+    """The reference's own CPU gold as it runs there (one thread, 16-bit
+    table popcounts, float32 scores, src/proc_gold_1.h:68-178) over the
+    leading rows of the panel, extrapolated to the whole scan the way its
+    progress estimate does (src/proc_gold_1.h:140-153): the rows j < J hold
+    J(J+1)/2 of the R(R+1)/2 row pairs."""
+    R = bits.shape[0]
+    panel = orc.Panel(bits, valid, m0, m1, n_samples)
+    J = 64
+    _, sec = orc.gold_scan_ref_f32(panel, topk, J)
+    J = int(min(R, J * (target_s / max(sec, 1e-3)) ** 0.5))
+    _, sec = orc.gold_scan_ref_f32(panel, topk, J)
+    done = J * (J + 1) // 2
+    total_s = sec * (R * (R + 1) / 2) / done
+    return {"value": 5 * R * (R + 1) / 2 / total_s, "unit": "evals/s", "cores": 1,
+            "kind": "port",
+            "sample": "single-thread gold scan as the reference runs it (16-bit table popcounts, "
+                      "float32 scores) over the first %d of %d rows (%d row pairs) in %.1f s; "
+                      "whole scan extrapolated to %.0f s by the pair count" % (
+                          J, R, done, sec, total_s)}
 
 
 def run_mix(args):
@@ -252,14 +280,20 @@ def run_serve(args):
     st = srv.stats()
     lat = sorted(r[1]["ms_scan"] for r in results)
     evals = sum(r[1]["evals"] for r in results)
-    # a sample of responses against the same scan run alone
-    identical = True
-    for i in [k for k, r in enumerate(reqs) if r[4] == "mix"][:4]:
-        p = reqs[i][5]
-        with ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"]) as h:
-            alone, _ = h.scan(reqs[i][1], reqs[i][2], topmost=100, ffun=args.ffun, count_path=3)
-        identical &= bool(np.array_equal(alone, results[i][0]))
     srv.close()
+    # every response (scans of the mix and re-scans of the resident panel)
+    # against the same scan run alone
+    identical, checked = True, 0
+    handles = {}
+    for i, (pid, m0, m1, _, kind, p) in enumerate(reqs):
+        if pid not in handles:
+            src = big if kind == "rescan" else p
+            handles[pid] = ge.Panel.from_bed(src["payload"], src["n_var"], src["n_samples"])
+        alone, _ = handles[pid].scan(m0, m1, topmost=100, ffun=args.ffun, count_path=3)
+        identical &= bool(np.array_equal(alone, results[i][0]))
+        checked += 1
+    for h in handles.values():
+        h.close()
     line = {
         "metric": "SNP-pair x Boolean-fn evals/s served under Poisson arrivals; p50/p99 request latency",
         "value": evals / wall, "unit": "evals/s", "n_gpus": 1, "steps": len(reqs), "warmup": 4,
@@ -275,6 +309,7 @@ def run_serve(args):
         "gpu_busy_frac": (st["busy_ms"] - base["busy_ms"]) / (1000.0 * wall),
         "rounds": st["rounds"] - base["rounds"], "max_active": st["max_active"],
         "failed": st["failed"], "responses_identical_to_alone": identical,
+        "responses_checked": checked,
         "roofline": None, "cpu_baseline": None, "gen_s": gen_s,
         "autotune_s": tune_s if args.autotune else None,
     }
@@ -309,8 +344,44 @@ def cli_wall(p, args):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def run_cpuref(args):
+    """The reference's CPU gold as it runs there (one thread, 16-bit table
+    popcounts, float32 scores) on the small workload (1,024 variants x 256
+    samples, whole scan, top-100) and on the first 2,000 variants of the
+    medium workload (whole subset scan, top-1,000), the latter extrapolated
+    to all 20,000 variants by the row-pair count.  CPU only."""
+    import shutil
+    import tempfile
+
+    from gepicount_amd import synth
+    from oracle import oracle as orc
+    out = {"metric": "reference-style CPU gold scan, one thread", "unit": "evals/s", "cores": 1,
+           "kind": "port", "runs": []}
+    d = tempfile.mkdtemp(prefix="gepi_cpuref_")
+    try:
+        for name, n_var, topk, full_var in (("small", None, 100, 1024), ("medium", 2000, 1000, 20000)):
+            p = synth.make_panel(name, n_var=n_var)
+            root = os.path.join(d, name)
+            synth.write_plink(root, p)
+            pan = orc.read_plink(root)
+            _, sec = orc.gold_scan_ref_f32(pan, topk)
+            R = pan.n_rows
+            ev = 5 * R * (R + 1) // 2
+            Rf = 2 * full_var
+            out["runs"].append({"workload": name, "variants": pan.n_rows // 2,
+                                "samples": pan.n_samples, "topk": topk, "seconds": sec,
+                                "evals_per_s": ev / sec,
+                                "extrapolated_full_s": sec * (Rf * (Rf + 1) / 2) / (R * (R + 1) / 2)})
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "cpuref":
+        run_cpuref(args)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

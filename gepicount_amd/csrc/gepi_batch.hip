@@ -422,7 +422,7 @@ int Engine::add(ScanItem *it) {
         return GEPI_OK;
     }
     int rc;
-    if ((rc = ensure_tables(tabs_, it->n0 + it->n1 + 1)) != GEPI_OK) {
+    if ((rc = ensure_tables(tabs_, it->n0 + it->n1 + 1, stream)) != GEPI_OK) {
         it->status = rc;
         it->done = true;
         return rc;

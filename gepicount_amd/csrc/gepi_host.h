@@ -83,7 +83,7 @@ int32_t device_cus(int device);
 int f4_wg_per_cu(int dflt);
 
 /* Grows the t*log2(t) and ln(t!) tables to cover t <= tmax. */
-int ensure_tables(DeviceTables &tabs, int32_t tmax);
+int ensure_tables(DeviceTables &tabs, int32_t tmax, hipStream_t stream);
 
 template <typename T>
 int grow(T **ptr, size_t *have, size_t need) {

@@ -104,11 +104,13 @@ int set_device(int device) {
     return GEPI_OK;
 }
 
-int ensure_tables(DeviceTables &tabs, int32_t tmax) {
+int ensure_tables(DeviceTables &tabs, int32_t tmax, hipStream_t stream) {
     if (tmax <= tabs.tmax) return GEPI_OK;
     int32_t n = tmax + 1;
     std::vector<double> hi(n), lo(n), lf(n);
     gepi_build_tables(n, hi.data(), lo.data(), lf.data());
+    /* the old tables may still be read by work queued on the stream */
+    HIP_TRY(hipStreamSynchronize(stream));
     hipFree(tabs.lhi);
     hipFree(tabs.llo);
     hipFree(tabs.lfact);
@@ -116,9 +118,14 @@ int ensure_tables(DeviceTables &tabs, int32_t tmax) {
     HIP_TRY(hipMalloc(&tabs.lhi, sizeof(double) * n));
     HIP_TRY(hipMalloc(&tabs.llo, sizeof(double) * n));
     HIP_TRY(hipMalloc(&tabs.lfact, sizeof(double) * n));
-    HIP_TRY(hipMemcpy(tabs.lhi, hi.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(tabs.llo, lo.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(tabs.lfact, lf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    /* copies on the stream that will read the tables (a null-stream copy is
+     * not ordered with a non-blocking stream); the host vectors live until
+     * the synchronisation below */
+    HIP_TRY(hipMemcpyAsync(tabs.lhi, hi.data(), sizeof(double) * n, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(tabs.llo, lo.data(), sizeof(double) * n, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(tabs.lfact, lf.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     tabs.tmax = tmax;
     return GEPI_OK;
 }
@@ -289,7 +296,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const int64_t pairs_per_tile = path == 3 ? (int64_t)TM * TB4 : (int64_t)tv * tv;
     const int32_t n_ks = path == 3 ? wused / 4 : wused / 2;
 
-    if ((rc = ensure_tables(p->tabs, n0 + n1 + 1)) != GEPI_OK) return rc;
+    if ((rc = ensure_tables(p->tabs, n0 + n1 + 1, p->stream)) != GEPI_OK) return rc;
     if (path != 3 &&
         (rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK)
         return rc;
@@ -576,8 +583,14 @@ int upload_planes(gepi_panel *p, const uint64_t *bits, const uint64_t *valid, in
             o[2 * words + w] = v1[w];
         }
     }
-    if (n_var > 0 && words > 0)
-        HIP_TRY(hipMemcpy(p->d_nat, nat.data(), nat.size() * 8, hipMemcpyHostToDevice));
+    /* on the panel's stream, after panel_alloc's zero fill of the same
+     * buffer (a null-stream copy would not be ordered with that non-blocking
+     * stream); nat lives until the synchronisation */
+    if (n_var > 0 && words > 0) {
+        HIP_TRY(hipMemcpyAsync(p->d_nat, nat.data(), nat.size() * 8, hipMemcpyHostToDevice,
+                               p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+    }
     return GEPI_OK;
 }
 
@@ -805,8 +818,12 @@ extern "C" int gepi_panel_download(gepi_panel *p, uint64_t *bit_data, uint64_t *
     if ((rc = set_device(p->device)) != GEPI_OK) return rc;
     const int32_t W = p->w64;
     std::vector<uint64_t> nat((size_t)p->n_var * 3 * W);
-    if (!nat.empty())
-        HIP_TRY(hipMemcpy(nat.data(), p->d_nat, nat.size() * 8, hipMemcpyDeviceToHost));
+    if (!nat.empty()) {
+        /* ordered after the panel's own queued work (its non-blocking stream) */
+        HIP_TRY(hipMemcpyAsync(nat.data(), p->d_nat, nat.size() * 8, hipMemcpyDeviceToHost,
+                               p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+    }
     for (int64_t v = 0; v < p->n_var; v++)
         for (int32_t w = 0; w < W; w++) {
             const uint64_t *o = nat.data() + (size_t)v * 3 * W;

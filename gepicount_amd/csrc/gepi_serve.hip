@@ -65,7 +65,16 @@ Key128 content_key(const uint8_t *p, size_t n, int64_t n_variants, int32_t n_sam
 struct Resident {
     gepi_panel *panel = nullptr;
     Key128 key{};
+    bool keyed = false;        /* by_key maps key to this panel */
+    std::vector<uint8_t> bed;  /* the payload, compared on a key hit */
+    int64_t n_variants = 0;
+    int32_t n_samples = 0;
     int64_t refs = 0; /* owners (add_panel calls) + requests in flight */
+
+    bool same(const uint8_t *p, size_t n, int64_t nv, int32_t ns) const {
+        return nv == n_variants && ns == n_samples && n == bed.size() &&
+               (n == 0 || memcmp(p, bed.data(), n) == 0);
+    }
 };
 
 struct Request {
@@ -115,7 +124,7 @@ struct gepi_server {
     }
 
     void drop_panel_locked(std::map<uint64_t, Resident>::iterator it) {
-        by_key.erase(it->second.key);
+        if (it->second.keyed) by_key.erase(it->second.key);
         gepi_panel_destroy(it->second.panel);
         panels.erase(it);
     }
@@ -229,36 +238,45 @@ extern "C" int gepi_server_add_panel_bed(gepi_server *s, const uint8_t *bed_payl
     if (n_variants < 0 || n_samples < 0) return gepi_fail(GEPI_ERR_ARG, "negative panel size");
     const size_t bytes = (size_t)n_variants * (size_t)((n_samples + 3) / 4);
     if (bytes && !bed_payload) return gepi_fail(GEPI_ERR_ARG, "null payload");
-    const Key128 key = content_key(bed_payload, bytes, n_variants, n_samples);
+    Key128 key = content_key(bed_payload, bytes, n_variants, n_samples);
+    /* test hook: every payload gets one key, so that distinct payloads collide */
+    if (const char *env = getenv("GEPI_SERVE_KEY_COLLIDE"); env && atoi(env) == 1) key = Key128{1, 1};
+    /* A key hit is reused only when the payload is byte-identical: two
+     * payloads with one key (a hash collision) get separate panels. */
+    auto reuse_locked = [&](uint64_t *id) {
+        auto it = s->by_key.find(key);
+        if (it == s->by_key.end()) return false;
+        Resident &r = s->panels[it->second];
+        if (!r.same(bed_payload, bytes, n_variants, n_samples)) return false;
+        r.refs++;
+        *id = it->second;
+        return true;
+    };
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) return gepi_fail(GEPI_ERR_DEVICE, "server stopped (error %d)", s->device_error);
-        auto it = s->by_key.find(key);
-        if (it != s->by_key.end()) {
-            s->panels[it->second].refs++;
-            *panel_id = it->second;
-            return GEPI_OK;
-        }
+        if (reuse_locked(panel_id)) return GEPI_OK;
     }
     gepi_panel *p = nullptr;
     int rc = gepi_panel_create_bed(bed_payload, n_variants, n_samples, s->par.device, &p);
     if (rc != GEPI_OK) return rc;
+    Resident r;
+    r.bed.assign(bed_payload, bed_payload + bytes);
     std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->by_key.find(key);
-    if (it != s->by_key.end()) { /* added by another thread meanwhile */
+    if (reuse_locked(panel_id)) { /* added by another thread meanwhile */
         gepi_panel_destroy(p);
-        s->panels[it->second].refs++;
-        *panel_id = it->second;
         return GEPI_OK;
     }
     uint64_t id = key.a ? key.a : 1;
     while (s->panels.count(id)) id++; /* keep ids unique if the first halves collide */
-    Resident r;
     r.panel = p;
     r.key = key;
+    r.keyed = s->by_key.count(key) == 0;
+    r.n_variants = n_variants;
+    r.n_samples = n_samples;
     r.refs = 1;
-    s->panels[id] = r;
-    s->by_key[key] = id;
+    if (r.keyed) s->by_key[key] = id;
+    s->panels[id] = std::move(r);
     *panel_id = id;
     return GEPI_OK;
 }

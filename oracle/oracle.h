@@ -72,6 +72,10 @@ const char *orc_last_error(void);
 /* ---- scoring ----------------------------------------------------------- */
 /* Exactly-rounded FP64 -I(F;C) for counts (w, x, y, z); 0 when y+z == 0. */
 double orc_entropy_exact(int w, int x, int y, int z);
+/* The same sum in plain double from the rounded table: |approx - exact| is
+ * below 9 * log2(m) * 2^-50 (< 1e-12 for m < 2^31).  The gold scan uses it
+ * only to skip the exact score of evaluations that cannot enter a full heap. */
+double orc_entropy_approx(int w, int x, int y, int z);
 /* The reference's host float32 expression (entropy_inline.h:70-76). */
 float orc_entropy_ref_f32(int w, int x, int y, int z);
 /* The reference's host Fisher log10 p (fisher_p.c:52-76), same argument order. */
@@ -88,6 +92,19 @@ double orc_log2(double v);
 int orc_gold_scan(const uint64_t *bits, const uint64_t *valid, const uint64_t *mask0,
                   const uint64_t *mask1, int64_t n_rows, int32_t words, int64_t topk,
                   int ffun, int nthreads, orc_result *out, int64_t *n_out);
+
+/* The same scan over the rows j in [j0, j1) only (every k <= j of each). */
+int orc_gold_scan_rows(const uint64_t *bits, const uint64_t *valid, const uint64_t *mask0,
+                       const uint64_t *mask1, int64_t n_rows, int32_t words, int64_t j0,
+                       int64_t j1, int64_t topk, int ffun, int nthreads, orc_result *out,
+                       int64_t *n_out);
+
+/* The reference's gold scan as it runs there (one thread, 16-bit table
+ * popcounts, float32 scores, proc_gold_1.h:68-178) over rows j < j_end;
+ * seconds = the scan's wall time.  For timing and its own pinning tests.   */
+int orc_gold_scan_ref_f32(const uint64_t *bits, const uint64_t *valid, const uint64_t *mask0,
+                          const uint64_t *mask1, int64_t n_rows, int32_t words, int64_t topk,
+                          int64_t j_end, orc_result *out, int64_t *n_out, double *seconds);
 
 /* Counts for every (row pair, function) in gold scan order: for j in [0,R),
  * k in [0,j], fun in [0,5): out[4*e + {0,1,2,3}] = {w, x, y, z}.           */

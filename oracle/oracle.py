@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# GEPI_ORACLE_LIB names another build of the same sources (for instance one
+# compiled for the local CPU to run a very large gold scan).
+LIB_PATH = os.environ.get("GEPI_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_FISHER_PATH = os.path.join(HERE, "_ref", "libfisher_ref.so")
 
 RESULT_DTYPE = np.dtype([("sA", "<i8"), ("sB", "<i8"), ("w", "<i4"), ("x", "<i4"),
@@ -51,6 +53,8 @@ def lib():
         l.orc_last_error.restype = ctypes.c_char_p
         l.orc_entropy_exact.argtypes = [ctypes.c_int] * 4
         l.orc_entropy_exact.restype = ctypes.c_double
+        l.orc_entropy_approx.argtypes = [ctypes.c_int] * 4
+        l.orc_entropy_approx.restype = ctypes.c_double
         l.orc_entropy_ref_f32.argtypes = [ctypes.c_int] * 4
         l.orc_entropy_ref_f32.restype = ctypes.c_float
         l.orc_fisher_ref.argtypes = [ctypes.c_int] * 4
@@ -63,6 +67,16 @@ def lib():
                                     ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.POINTER(ctypes.c_int64)]
         l.orc_gold_scan.restype = ctypes.c_int
+        l.orc_gold_scan_rows.argtypes = [u64p, u64p, u64p, u64p, ctypes.c_int64,
+                                         ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+        l.orc_gold_scan_rows.restype = ctypes.c_int
+        l.orc_gold_scan_ref_f32.argtypes = [u64p, u64p, u64p, u64p, ctypes.c_int64,
+                                            ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_double)]
+        l.orc_gold_scan_ref_f32.restype = ctypes.c_int
         l.orc_all_counts.argtypes = [u64p, u64p, u64p, u64p, ctypes.c_int64, ctypes.c_int32,
                                      ctypes.c_void_p]
         l.orc_all_counts.restype = ctypes.c_int
@@ -136,6 +150,42 @@ def gold_scan(panel, topk, ffun=0, nthreads=0):
     if rc != 0:
         raise ValueError("orc_gold_scan failed")
     return out[:n.value].copy()
+
+
+def gold_scan_rows(panel, j0, j1, topk, ffun=0, nthreads=0):
+    """The gold scan over rows j in [j0, j1) (every k <= j): the top-k of that
+    part of the scan, sorted by canonical key."""
+    l = lib()
+    R, W = panel.n_rows, panel.words
+    total = 5 * (j1 * (j1 + 1) // 2 - j0 * (j0 + 1) // 2)
+    out = np.zeros(max(1, min(topk, total)), dtype=RESULT_DTYPE)
+    n = ctypes.c_int64(0)
+    m0 = panel.mask_ctrl if panel.mask_ctrl is not None else np.zeros(max(W, 1), np.uint64)
+    m1 = panel.mask_case if panel.mask_case is not None else np.zeros(max(W, 1), np.uint64)
+    rc = l.orc_gold_scan_rows(_u64p(panel.bits), _u64p(panel.valid), _u64p(m0), _u64p(m1), R, W,
+                              j0, j1, topk, ffun, nthreads, out.ctypes.data, ctypes.byref(n))
+    if rc != 0:
+        raise ValueError("orc_gold_scan_rows failed")
+    return out[:n.value].copy()
+
+
+def gold_scan_ref_f32(panel, topk, j_end=None):
+    """The reference's single-thread float32 gold scan over rows j < j_end.
+    Returns (results with obj = the float32 score, seconds)."""
+    l = lib()
+    R, W = panel.n_rows, panel.words
+    j_end = R if j_end is None else int(j_end)
+    out = np.zeros(max(1, topk), dtype=RESULT_DTYPE)
+    n = ctypes.c_int64(0)
+    sec = ctypes.c_double(0.0)
+    m0 = panel.mask_ctrl if panel.mask_ctrl is not None else np.zeros(max(W, 1), np.uint64)
+    m1 = panel.mask_case if panel.mask_case is not None else np.zeros(max(W, 1), np.uint64)
+    rc = l.orc_gold_scan_ref_f32(_u64p(panel.bits), _u64p(panel.valid), _u64p(m0), _u64p(m1), R,
+                                 W, topk, j_end, out.ctypes.data, ctypes.byref(n),
+                                 ctypes.byref(sec))
+    if rc != 0:
+        raise ValueError("orc_gold_scan_ref_f32 failed")
+    return out[:n.value].copy(), sec.value
 
 
 def all_counts(panel):

@@ -20,6 +20,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Far above the approximate score's error bound (orc_entropy_approx). */
+#define ORC_APPROX_MARGIN 1e-9
+
 typedef struct {
     double obj;
     uint64_t idx; /* sA << 32 | sB << 3 | fun : orders like (sA, sB, fun) */
@@ -126,11 +129,22 @@ int orc_all_counts(const uint64_t *bits, const uint64_t *valid, const uint64_t *
 int orc_gold_scan(const uint64_t *bits, const uint64_t *valid, const uint64_t *mask0,
                   const uint64_t *mask1, int64_t R, int32_t W, int64_t topk, int ffun,
                   int nthreads, orc_result *out, int64_t *n_out) {
+    return orc_gold_scan_rows(bits, valid, mask0, mask1, R, W, 0, R, topk, ffun, nthreads, out,
+                              n_out);
+}
+
+/* The gold scan restricted to the rows j in [j0, j1) (all k <= j of each).
+ * The top-k of a whole scan is the top-k of the union of the top-k lists of
+ * any partition of [0, R) into row ranges, because the key is total.       */
+int orc_gold_scan_rows(const uint64_t *bits, const uint64_t *valid, const uint64_t *mask0,
+                       const uint64_t *mask1, int64_t R, int32_t W, int64_t j0, int64_t j1,
+                       int64_t topk, int ffun, int nthreads, orc_result *out, int64_t *n_out) {
     *n_out = 0;
     if (R < 0 || W < 0 || topk < 0 || (ffun != ORC_FFUNC_ENTROPY && ffun != ORC_FFUNC_FISHERPV))
         return -1;
     if (R >= (1LL << 29)) return -1;
-    int64_t total = ORC_NUM_FUNCS * (R * (R + 1) / 2);
+    if (j0 < 0 || j1 > R || j0 > j1) return -1;
+    int64_t total = ORC_NUM_FUNCS * (j1 * (j1 + 1) / 2 - j0 * (j0 + 1) / 2);
     int64_t k = topk < total ? topk : total;
     if (k == 0) return 0;
     int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -142,8 +156,8 @@ int orc_gold_scan(const uint64_t *bits, const uint64_t *valid, const uint64_t *m
         heaps[t].k = k;
     }
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 4)
-    for (int64_t jj = 0; jj < R; jj++) {
-        int64_t j = R - 1 - jj; /* long rows first */
+    for (int64_t jj = 0; jj < j1 - j0; jj++) {
+        int64_t j = j1 - 1 - jj; /* long rows first */
         heap_t *h = &heaps[omp_get_thread_num()];
         for (int64_t kk = 0; kk <= j; kk++) {
             int32_t cnt[ORC_NUM_FUNCS][4];
@@ -155,6 +169,11 @@ int orc_gold_scan(const uint64_t *bits, const uint64_t *valid, const uint64_t *m
                 e.x = cnt[f][1];
                 e.y = cnt[f][2];
                 e.z = cnt[f][3];
+                /* A full heap only admits keys below its top; an entropy more
+                 * than ORC_APPROX_MARGIN above the top's cannot be one. */
+                if (ffun == ORC_FFUNC_ENTROPY && h->n == h->k &&
+                    orc_entropy_approx(e.w, e.x, e.y, e.z) > h->v[0].obj + ORC_APPROX_MARGIN)
+                    continue;
                 e.obj = (ffun == ORC_FFUNC_ENTROPY) ? orc_entropy_exact(e.w, e.x, e.y, e.z)
                                                     : orc_fisher_rank(e.w, e.x, e.y, e.z);
                 e.idx = ((uint64_t)j << 32) | ((uint64_t)kk << 3) | (uint64_t)f;

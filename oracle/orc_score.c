@@ -114,6 +114,16 @@ double orc_entropy_exact(int w, int x, int y, int z) {
     return r;
 }
 
+double orc_entropy_approx(int w, int x, int y, int z) {
+    int64_t m = (int64_t)y + z;
+    if (m == 0) return 0.0;
+    int64_t xx = (int64_t)w + x;
+    ltable_ensure(m);
+    double s = g_lhi[y] + g_lhi[z] + g_lhi[xx] + g_lhi[m - xx];
+    s -= g_lhi[w] + g_lhi[y - w] + g_lhi[x] + g_lhi[z - x] + g_lhi[m];
+    return s / (double)m;
+}
+
 float orc_entropy_ref_f32(int w, int x, int y, int z) {
     float m = (float)(y + z);
     float xx = (float)(w + x);

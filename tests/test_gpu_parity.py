@@ -227,6 +227,27 @@ def test_bed_decode_equals_host_planes():
     assert np.array_equal(valid, d.valid_data)
 
 
+def test_panel_create_download_round_trip():
+    """Panels created from planes (the path of gepi_process_gpu_1) and from
+    .bed bytes read back exactly, many tiny panels in a row: the zero fill,
+    the upload and the read-back all run on the panel's own stream."""
+    print("case panel round trip", flush=True)
+    rng = np.random.default_rng(77)
+    for i in range(40):
+        n_var = int(rng.integers(1, 7))
+        n = int(rng.integers(1, 200))
+        codes = orc.random_codes(rng, n_var, n, 0.1)
+        bits, valid = orc.planes_from_codes(codes)
+        with ge.Panel.from_planes(bits, valid, n) as p:
+            b2, v2 = p.download()
+        assert np.array_equal(b2, bits & valid), ("planes", i, n_var, n)
+        assert np.array_equal(v2, valid), ("planes valid", i, n_var, n)
+        with ge.Panel.from_bed(synth.pack_codes(codes), n_var, n) as p:
+            b3, v3 = p.download()
+        assert np.array_equal(b3, bits & valid), ("bed", i, n_var, n)
+        assert np.array_equal(v3, valid), ("bed valid", i, n_var, n)
+
+
 def test_resident_panel_rescans_new_phenotypes():
     bits, valid, m0, m1 = random_panel(31, 90, 500, 0.02)
     with ge.Panel.from_planes(bits, valid, 500) as panel:
@@ -346,6 +367,19 @@ def test_topk_zero_and_more_than_all():
     assert len(got) == 0
     got, _ = ge.process_gpu_1(bits, valid, m0, m1, 70, topmost=10**6)
     assert len(got) == 5 * 8 * 9 // 2
+
+
+def test_cli_report_equals_hand_derived_kat_report():
+    """gepicount -i data/test -N 10 prints the hand-written report (every
+    column derived from the .bed/.bim/.fam bytes; FPv from the reference's
+    fisher_p.c) byte for byte on stderr."""
+    from test_oracle import KAT_REPORT_N10
+    exe = os.path.join(ROOT, "gepicount_amd", "gepicount")
+    r = subprocess.run([exe, "-i", os.path.join(DATA, "test"), "-N", "10"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    start = r.stderr.index(KAT_REPORT_N10[:40])
+    assert r.stderr[start:start + len(KAT_REPORT_N10)] == KAT_REPORT_N10
 
 
 def test_cli_report_equals_oracle_report():

@@ -1,6 +1,6 @@
-"""Size-independent properties of the scan at the full large size
-(100,000 variants x 8,192 samples, 1e11 evaluations), where the oracle's
-gold scan of the whole panel is out of reach:
+"""The scan at the full large size (100,000 variants x 8,192 samples, 1e11
+evaluations): the entropy top-10,000 against the committed result of the
+oracle's gold scan of the whole panel, and size-independent properties:
 
 - every returned record's counts, recomputed with numpy popcounts from the
   panel, and its objective, recomputed by the oracle, are bit-identical;
@@ -11,6 +11,9 @@ gold scan of the whole panel is out of reach:
   inside the subset is in the oracle's top-k of the subset;
 - the three count paths return identical lists.
 """
+import hashlib
+import os
+
 import numpy as np
 import pytest
 
@@ -19,6 +22,9 @@ from gepicount_amd import synth
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                    "large_entropy_top10000.npz")
 
 
 def order_key(obj):
@@ -40,7 +46,10 @@ def large():
     m0, m1 = synth.masks_from_pheno(p["pheno"])
     panel = ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"])
     bits, valid = panel.download()
-    yield dict(panel=panel, m0=m0, m1=m1, bits=bits, valid=valid, n=p["n_samples"])
+    bed_sha = hashlib.sha256(np.ascontiguousarray(p["payload"]).tobytes()).hexdigest()
+    pheno_sha = hashlib.sha256(np.asarray(p["pheno"], np.int64).tobytes()).hexdigest()
+    yield dict(panel=panel, m0=m0, m1=m1, bits=bits, valid=valid, n=p["n_samples"],
+               bed_sha=bed_sha, pheno_sha=pheno_sha)
     panel.close()
 
 
@@ -108,6 +117,25 @@ def test_large_entropy_top10000(large):
     assert st["count_path"] == 3
     check_records(large, res, 10000, 0)
     check_subset_completeness(large, res, 10000, 0)
+
+
+def test_large_entropy_top10000_equals_gold(large):
+    """The whole large scan against the oracle's gold scan of all 1e11
+    evaluations (tests/golden/make_large_gold.py): the same 10,000 records,
+    record by record, in the same order, with the same counts and score bits."""
+    print("case large entropy top-10000 vs the full gold scan", flush=True)
+    g = np.load(GOLD)
+    assert str(g["bed_sha256"]) == large["bed_sha"], "regenerated panel differs from the gold's"
+    assert str(g["pheno_sha256"]) == large["pheno_sha"], "regenerated phenotypes differ"
+    k = int(g["topk"])
+    res, st = large["panel"].scan(large["m0"], large["m1"], topmost=k)
+    assert len(res) == k
+    for f in ("sA", "sB", "fun", "w", "x", "y", "z"):
+        if not np.array_equal(res[f], g[f].astype(res[f].dtype)):
+            i = int(np.flatnonzero(res[f] != g[f].astype(res[f].dtype))[0])
+            raise AssertionError("field %s differs first at rank %d: got %r, gold %r" % (
+                f, i, res[i], {h: g[h][i] for h in ("sA", "sB", "fun", "w", "x", "y", "z")}))
+    np.testing.assert_array_equal(res["obj"].view(np.uint64), g["obj_bits"])
 
 
 def test_large_count_paths_agree(large):

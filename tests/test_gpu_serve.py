@@ -150,6 +150,39 @@ def test_server_errors_and_panel_release():
             srv.release_panel(pid)
 
 
+def test_server_key_collision_keeps_panels_apart(monkeypatch):
+    """With every payload forced onto one content key, distinct payloads of
+    one shape still get distinct resident panels (the bytes are compared on
+    a key hit), and identical bytes still share one."""
+    print("case server key collision", flush=True)
+    monkeypatch.setenv("GEPI_SERVE_KEY_COLLIDE", "1")
+    rng = np.random.default_rng(31)
+    n_var, n = 40, 300
+    panels = []
+    for _ in range(2):
+        codes = orc.random_codes(rng, n_var, n, 0.02)
+        bits, valid = orc.planes_from_codes(codes)
+        panels.append(dict(payload=synth.pack_codes(codes), n_var=n_var, n=n, bits=bits,
+                           valid=valid))
+    cls = np.where(rng.random(n) < 0.5, 1, 0)
+    m0, m1 = orc.masks_from_classes(cls)
+    with ge.Server() as srv:
+        a = srv.add_panel_bed(panels[0]["payload"], n_var, n)
+        b = srv.add_panel_bed(panels[1]["payload"], n_var, n)
+        assert a != b
+        assert srv.add_panel_bed(panels[0]["payload"], n_var, n) == a
+        assert srv.stats()["panels"] == 2
+        got = [srv.wait(srv.submit(pid, m0, m1, topmost=300)) for pid in (a, b)]
+        # releasing the keyed panel leaves the other one usable
+        srv.release_panel(a)
+        srv.release_panel(a)
+        again = srv.wait(srv.submit(b, m0, m1, topmost=300))
+    for (res, _, status), p in zip(got + [again], panels + [panels[1]]):
+        assert status == 0
+        ref = orc.gold_scan(orc.Panel(p["bits"], p["valid"], m0, m1, n), 300)
+        assert_same(res, ref, "collided panel")
+
+
 def test_server_autotuned():
     """A server whose engine ran the autotuner at creation serves results
     identical to the oracle's."""

@@ -217,3 +217,147 @@ def test_reader_rejects_bad_inputs(tmp_path):
         orc.read_plink(write("bim", fam, bim.replace("\t0\t1\t", "\tzz\t1\t"), bed))
     with pytest.raises(ValueError, match="cannot open"):
         orc.read_plink(str(tmp_path / "absent"))
+
+
+# The data/test report for -N 10, written out by hand: ranks and (sA, sB, fun)
+# from the Appendix-A scores under the canonical key (nine evaluations tie at
+# -I = -0.4591479170, then (1, 0, NAND) leads the -0.1908745046 group), row
+# names from data/test.bim (row 2i = column 5 allele, row 2i+1 = column 6),
+# cells (w, y-w, x, z-x) from the Appendix-A count table, Obj as float %g.
+# The FPv column is the reference's own fisher_pval (src/misc/fisher_p.c,
+# compiled unmodified into oracle/_ref) of those cells: -0.39794 for the
+# 3-1/1-3 split tables (two-sided p = 0.4), 0 for (1,2,0,3).
+KAT_REPORT_N10 = """
+Rank:\tidxA[id allele chr:pos]\tFunc\tidxB[id allele chr:pos]\tObj\tFPv\t(c00,c01,c10,c11)
+0:\t2[snp2 G 1:2]\tAND\t0[snp1 A 1:1]\t-0.459148\t-0.39794\t(2,1,0,3)
+1:\t2[snp2 G 1:2]\tNAND\t1[snp1 C 1:1]\t-0.459148\t-0.39794\t(0,3,2,1)
+2:\t2[snp2 G 1:2]\tAND\t2[snp2 G 1:2]\t-0.459148\t-0.39794\t(3,0,1,2)
+3:\t2[snp2 G 1:2]\tNANDN\t2[snp2 G 1:2]\t-0.459148\t-0.39794\t(0,3,2,1)
+4:\t3[snp2 T 1:2]\tANDN\t0[snp1 A 1:1]\t-0.459148\t-0.39794\t(0,3,2,1)
+5:\t3[snp2 T 1:2]\tXOR\t0[snp1 A 1:1]\t-0.459148\t-0.39794\t(0,3,2,1)
+6:\t3[snp2 T 1:2]\tAND\t1[snp1 C 1:1]\t-0.459148\t-0.39794\t(1,2,3,0)
+7:\t3[snp2 T 1:2]\tXOR\t1[snp1 C 1:1]\t-0.459148\t-0.39794\t(2,1,0,3)
+8:\t3[snp2 T 1:2]\tANDN\t2[snp2 G 1:2]\t-0.459148\t-0.39794\t(0,3,2,1)
+9:\t1[snp1 C 1:1]\tNAND\t0[snp1 A 1:1]\t-0.190875\t0\t(1,2,0,3)
+"""
+
+
+def test_kat_report_text_hand_derived():
+    """The oracle's gold command line and its report_lines reproduce the
+    hand-written data/test report byte for byte."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(orc.HERE), "oracle", "gepicount_gold")
+    r = subprocess.run([exe, "-i", os.path.join(DATA, "test"), "-N", "10"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stderr == KAT_REPORT_N10
+    p = orc.read_plink(os.path.join(DATA, "test"))
+    tops = orc.gold_scan(p, 10)
+    assert "\n".join(orc.report_lines(p, tops)) + "\n" == KAT_REPORT_N10
+
+
+def test_reader_thinhap1_equals_planes_from_codes():
+    """data/thinhap1 (517 missing calls): the oracle reader's planes against
+    the per-sample decoding of the raw .bed codes (BinPlinkReader.h:171-201)."""
+    root = os.path.join(DATA, "thinhap1")
+    p = orc.read_plink(root)
+    raw = np.fromfile(root + ".bed", np.uint8)
+    assert raw[:3].tolist() == [0x6C, 0x1B, 0x01]
+    n, N = p.n_samples, p.n_rows // 2
+    payload = raw[3:].reshape(N, (n + 3) // 4)
+    q = np.stack([(payload >> (2 * i)) & 3 for i in range(4)], axis=-1).reshape(N, -1)[:, :n]
+    assert int((q == 1).sum()) == 517
+    bits, valid = orc.planes_from_codes(q)
+    assert np.array_equal(p.valid, valid)
+    assert np.array_equal(p.bits, bits)
+    assert p.total_missing == 517
+
+
+def test_entropy_approx_within_bound():
+    """The gold scan's skip test: the plain-double sum stays within 1e-12 of
+    the exact score over the golden tuples and random large tables."""
+    with open(os.path.join(GOLDEN, "entropy_vectors.json")) as f:
+        vec = json.load(f)
+    tuples = [tuple(v[:4]) for v in vec["rows"]]
+    rng = np.random.default_rng(3)
+    for _ in range(20000):
+        y, z = int(rng.integers(0, 9000)), int(rng.integers(0, 9000))
+        tuples.append((int(rng.integers(0, y + 1)), int(rng.integers(0, z + 1)), y, z))
+    for w, x, y, z in tuples:
+        a, e = orc.lib().orc_entropy_approx(w, x, y, z), orc.entropy_exact(w, x, y, z)
+        assert abs(a - e) < 1e-12, (w, x, y, z, a, e)
+
+
+def test_gold_scan_rows_partition():
+    """Row ranges partition the gold scan: the top-k of the union of the
+    ranges' top-k lists is the whole scan's top-k."""
+    rng = np.random.default_rng(11)
+    codes = orc.random_codes(rng, 40, 300, 0.02)
+    cls = rng.choice([0, 1], size=300)
+    bits, valid = orc.planes_from_codes(codes)
+    m0, m1 = orc.masks_from_classes(cls)
+    panel = orc.Panel(bits, valid, m0, m1, 300)
+    whole = orc.gold_scan(panel, 500)
+    parts = [orc.gold_scan_rows(panel, a, b, 500) for a, b in ((0, 30), (30, 57), (57, 80))]
+    allp = np.concatenate(parts)
+    b = allp["obj"].view(np.uint64)
+    hi = np.where(b >> np.uint64(63), ~b, b | np.uint64(1 << 63))
+    lo = (allp["sA"].astype(np.uint64) << np.uint64(32)) | \
+        (allp["sB"].astype(np.uint64) << np.uint64(3)) | allp["fun"].astype(np.uint64)
+    merged = allp[np.lexsort((lo, hi))][:500]
+    assert merged.tobytes() == whole.tobytes()
+    # every range's records lie in its rows
+    for (a, bnd), r in zip(((0, 30), (30, 57), (57, 80)), parts):
+        assert np.all((r["sA"] >= a) & (r["sA"] < bnd))
+
+
+def test_gold_scan_ref_f32_is_the_stable_float_ranking():
+    """The reference-style single-thread gold (16-bit table popcounts, float32
+    scores, 2k buffer with a strict limit) returns the k smallest float32
+    scores in scan order among ties, with brute-force counts, also when the
+    scan stops early at j_end."""
+    rng = np.random.default_rng(17)
+    codes = orc.random_codes(rng, 9, 150, 0.05)
+    cls = rng.choice([0, 1, -1], size=150, p=[0.5, 0.45, 0.05])
+    bits, valid = orc.planes_from_codes(codes)
+    m0, m1 = orc.masks_from_classes(cls)
+    panel = orc.Panel(bits, valid, m0, m1, 150)
+    cnt = orc.brute_counts(codes, cls)
+    R = bits.shape[0]
+    recs = []
+    e = 0
+    for j in range(R):
+        for k in range(j + 1):
+            for f in range(5):
+                w, x, y, z = (int(v) for v in cnt[e, f])
+                recs.append((np.float32(orc.entropy_ref_f32(w, x, y, z)), len(recs), j, k, f,
+                             w, x, y, z))
+            e += 1
+    for topk, j_end in ((7, R), (60, R), (60, 11), (5000, R)):
+        sub = [r for r in recs if r[2] < j_end]
+        ref = sorted(sub, key=lambda r: (r[0], r[1]))[:topk]
+        got, sec = orc.gold_scan_ref_f32(panel, topk, j_end)
+        assert sec >= 0.0
+        assert [(np.float32(g["obj"]), int(g["sA"]), int(g["sB"]), int(g["fun"]), int(g["w"]),
+                 int(g["x"]), int(g["y"]), int(g["z"])) for g in got] == \
+            [(r[0],) + r[2:] for r in ref]
+
+
+def test_large_gold_fixture_self_consistent():
+    """The committed gold top-10,000 of the large workload: keys strictly
+    increase, every score is the oracle's exact score of its counts, and the
+    counts fit the class sizes (5,734 controls, 2,458 cases)."""
+    g = np.load(os.path.join(GOLDEN, "large_entropy_top10000.npz"))
+    assert int(g["topk"]) == 10000 and len(g["sA"]) == 10000
+    assert len(str(g["bed_sha256"])) == 64 and len(str(g["pheno_sha256"])) == 64
+    b = g["obj_bits"]
+    hi = np.where(b >> np.uint64(63), ~b, b | np.uint64(1 << 63))
+    lo = (g["sA"].astype(np.uint64) << np.uint64(32)) | \
+        (g["sB"].astype(np.uint64) << np.uint64(3)) | g["fun"].astype(np.uint64)
+    assert np.all((hi[1:] > hi[:-1]) | ((hi[1:] == hi[:-1]) & (lo[1:] > lo[:-1])))
+    assert np.all(g["sA"] >= g["sB"]) and np.all(g["sA"] < 200000) and np.all(g["fun"] < 5)
+    w, x, y, z = (g[f].astype(np.int64) for f in "wxyz")
+    assert np.all((0 <= w) & (w <= y) & (y <= 5734) & (0 <= x) & (x <= z) & (z <= 2458))
+    exact = np.array([orc.entropy_exact(int(a), int(c), int(d), int(e))
+                      for a, c, d, e in zip(w, x, y, z)])
+    assert np.array_equal(exact.view(np.uint64), b)
