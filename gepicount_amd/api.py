@@ -27,7 +27,8 @@ RESULT_DTYPE = np.dtype([("sA", "<i8"), ("sB", "<i8"), ("w", "<i4"), ("x", "<i4"
 
 # Names of every entry point declared in include/gepicount.h.
 EXPORTS = ("gepi_params_init", "gepi_process_gpu_1", "gepi_panel_create", "gepi_panel_create_bed",
-           "gepi_panel_destroy", "gepi_scan", "gepi_panel_download", "gepi_panel_qa",
+           "gepi_panel_destroy", "gepi_scan", "gepi_scan_noalloc", "gepi_panel_download",
+           "gepi_panel_qa",
            "gepi_panel_filter", "gepi_plink_open",
            "gepi_plink_close", "gepi_plink_check_quality", "gepi_print_interactions",
            "gepi_fisher_pval", "gepi_score_table", "gepi_last_error", "gepi_version",
@@ -59,7 +60,8 @@ class ScanStats(ctypes.Structure):
                 ("evals", ctypes.c_int64), ("exact_evals", ctypes.c_int64),
                 ("bands", ctypes.c_int32), ("count_path", ctypes.c_int32),
                 ("n_class0", ctypes.c_int32), ("n_class1", ctypes.c_int32),
-                ("pend_overflows", ctypes.c_int32), ("cand_overflows", ctypes.c_int32)]
+                ("pend_overflows", ctypes.c_int32), ("cand_overflows", ctypes.c_int32),
+                ("bytes_allocated", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -150,6 +152,8 @@ def lib():
     l.gepi_panel_destroy.argtypes = [vp]
     l.gepi_scan.argtypes = [vp, u64p, u64p, ctypes.POINTER(Params), vp,
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ScanStats)]
+    l.gepi_scan_noalloc.argtypes = [vp, u64p, u64p, ctypes.POINTER(Params), vp,
+                                    ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ScanStats)]
     l.gepi_panel_download.argtypes = [vp, u64p, u64p]
     i32p, i64p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)
     l.gepi_panel_qa.argtypes = [vp, ctypes.c_float, ctypes.c_float, i32p, i32p, i64p, i64p]
@@ -302,6 +306,19 @@ class Panel:
         st = ScanStats()
         _check(lib().gepi_scan(self._h, _u64p(m0), _u64p(m1), ctypes.byref(par),
                                out.ctypes.data, ctypes.byref(n), ctypes.byref(st)))
+        return out[:n.value].copy(), st.as_dict()
+
+    def scan_noalloc(self, mask_ctrl, mask_case, topmost=1000000, ffun=GEPI_FFUNC_ENTROPY,
+                     count_path=0):
+        """The same scan through gepi_scan_noalloc: no device allocation, all
+        buffers in the library's fixed per-GPU scratch array (fp4 path)."""
+        m0, m1 = _masks(mask_ctrl, mask_case, self.words)
+        par = make_params(topmost, ffun, 0, count_path, 0)
+        out = _result_buffer(topmost, self.n_rows)
+        n = ctypes.c_int64(0)
+        st = ScanStats()
+        _check(lib().gepi_scan_noalloc(self._h, _u64p(m0), _u64p(m1), ctypes.byref(par),
+                                       out.ctypes.data, ctypes.byref(n), ctypes.byref(st)))
         return out[:n.value].copy(), st.as_dict()
 
     def qa(self, max_miss_snps=0.05, max_miss_samples=0.05):

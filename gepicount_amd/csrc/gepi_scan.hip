@@ -132,6 +132,10 @@ int ensure_tables(DeviceTables &tabs, int32_t tmax, hipStream_t stream) {
 
 } // namespace gepi_host
 
+/* The fixed scratch array of gepi_scan_noalloc (layout at bind_scratch).  */
+constexpr size_t SCRATCH_BYTES = (size_t)GEPI_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
 /* ------------------------------------------------------------------------ */
 /* panel                                                                     */
 
@@ -199,22 +203,120 @@ int validate_params(const gepi_params *par) {
 }
 
 /* Select the rank-th smallest key (1-based) among the count records of src. */
-void launch_select(gepi_panel *p, const Cand *src, uint32_t rank, unsigned long long count) {
-    k_sel_reset<<<1, 1, 0, p->stream>>>(p->d_state, rank);
+/* Device buffers of one scan: the panel's own (grown on demand) or carved
+ * from the fixed scratch array of gepi_scan_noalloc. */
+struct ScanBufs {
+    uint32_t *cls = nullptr;
+    uint64_t *clsk = nullptr;
+    int32_t *perm = nullptr;
+    CompressWord *cw = nullptr;
+    Cand *cand[2] = {nullptr, nullptr};
+    Pend *pend = nullptr;
+    ScanState *state = nullptr;
+    uint64_t *small = nullptr;
+    const double *lhi = nullptr, *llo = nullptr, *lfact = nullptr;
+};
+
+void launch_select(gepi_panel *p, const ScanBufs &b, const Cand *src, uint32_t rank,
+                   unsigned long long count) {
+    k_sel_reset<<<1, 1, 0, p->stream>>>(b.state, rank);
     unsigned long long blocks = (count + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     for (int pass = 0; pass < 16; pass++) {
-        k_sel_hist<<<(unsigned)blocks, 256, 0, p->stream>>>(src, p->d_state);
-        k_sel_pick<<<1, 1, 0, p->stream>>>(p->d_state);
+        k_sel_hist<<<(unsigned)blocks, 256, 0, p->stream>>>(src, b.state);
+        k_sel_pick<<<1, 1, 0, p->stream>>>(b.state);
     }
-    k_sel_gather<<<(unsigned)blocks, 256, 0, p->stream>>>(src, p->d_state, p->d_small);
-    k_sel_small<<<1, SEL_SMALL, 0, p->stream>>>(p->d_state, p->d_small);
+    k_sel_gather<<<(unsigned)blocks, 256, 0, p->stream>>>(src, b.state, b.small);
+    k_sel_small<<<1, SEL_SMALL, 0, p->stream>>>(b.state, b.small);
 }
 
-int read_state(gepi_panel *p, ScanState *s) {
-    HIP_TRY(hipMemcpyAsync(s, p->d_state, sizeof(ScanState), hipMemcpyDeviceToHost, p->stream));
+int read_state(gepi_panel *p, const ScanBufs &b, ScanState *s) {
+    HIP_TRY(hipMemcpyAsync(s, b.state, sizeof(ScanState), hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
+    return GEPI_OK;
+}
+
+/* ---- fixed scratch of gepi_scan_noalloc ---------------------------------
+ * `scratch` (file scope above): one array per GPU, held until the process
+ * ends.  Layout: the score tables
+ * (t log2 t high and low parts, ln t!) for t <= GEPI_MAX_SAMPLES + 1 at fixed
+ * offsets, the scan state, the select buffer, then the per-call regions
+ * (compression words, class layout, pending list, two candidate lists). In
+ * host code `scratch` is only a symbol: its device address comes from
+ * hipGetSymbolAddress, once per device.                                     */
+constexpr size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+constexpr int32_t SCR_TMAX = GEPI_MAX_SAMPLES + 1;
+constexpr size_t SCR_TAB = align256(sizeof(double) * (size_t)(SCR_TMAX + 1));
+constexpr size_t SCR_STATE = 3 * SCR_TAB;
+constexpr size_t SCR_SMALL = SCR_STATE + align256(sizeof(ScanState));
+constexpr size_t SCR_DYN = SCR_SMALL + align256(sizeof(uint64_t) * 2 * SEL_SMALL);
+static_assert(SCR_DYN < SCRATCH_BYTES / 8, "fixed scratch part too large");
+
+struct ScratchDevice {
+    unsigned char *ptr = nullptr; /* device address of `scratch` on this GPU */
+    int32_t tab_tmax = -1;        /* tables valid for t <= tab_tmax           */
+};
+ScratchDevice g_scratch[64];
+std::mutex g_scratch_mu;
+
+/* Binds b to the scratch array of p's device (current): tables grown to
+ * tmax, then regions for the compression words, the class layout and the
+ * lists.  Allocates nothing; fails with GEPI_ERR_LIMIT when they do not fit. */
+int bind_scratch(gepi_panel *p, ScanBufs &b, int32_t tmax, size_t cw_bytes, size_t clsk_bytes,
+                 unsigned long long cap, unsigned long long pcap) {
+    const int dev = p->device;
+    if (dev < 0 || dev >= 64) return gepi_fail(GEPI_ERR_DEVICE, "device %d out of range", dev);
+    const size_t need = SCR_DYN + align256(cw_bytes) + align256(clsk_bytes) +
+                        align256(sizeof(Pend) * pcap) + 2 * align256(sizeof(Cand) * cap);
+    if (need > SCRATCH_BYTES)
+        return gepi_fail(GEPI_ERR_LIMIT,
+                         "gepi_scan_noalloc: this scan needs %zu bytes of scratch, more than its "
+                         "%zu (class layout %zu, candidate lists %llu records); use gepi_scan",
+                         need, SCRATCH_BYTES, clsk_bytes, cap);
+    if (tmax > SCR_TMAX)
+        return gepi_fail(GEPI_ERR_LIMIT, "gepi_scan_noalloc: %d class samples exceed %d", tmax - 1,
+                         SCR_TMAX - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ScratchDevice &sd = g_scratch[dev];
+    if (!sd.ptr) {
+        void *q = nullptr;
+        hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !q)
+            return gepi_fail(GEPI_ERR_DEVICE, "scratch address on device %d: %s", dev,
+                             hipGetErrorString(e));
+        sd.ptr = static_cast<unsigned char *>(q);
+    }
+    unsigned char *base = sd.ptr;
+    double *lhi = reinterpret_cast<double *>(base);
+    double *llo = reinterpret_cast<double *>(base + SCR_TAB);
+    double *lf = reinterpret_cast<double *>(base + 2 * SCR_TAB);
+    if (tmax > sd.tab_tmax) {
+        /* the tables depend on t only: a longer prefix replaces a shorter one */
+        const int32_t n = tmax + 1;
+        std::vector<double> hi(n), lo(n), f(n);
+        gepi_build_tables(n, hi.data(), lo.data(), f.data());
+        HIP_TRY(hipMemcpyAsync(lhi, hi.data(), sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipMemcpyAsync(llo, lo.data(), sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipMemcpyAsync(lf, f.data(), sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+        sd.tab_tmax = tmax;
+    }
+    b.lhi = lhi;
+    b.llo = llo;
+    b.lfact = lf;
+    b.state = reinterpret_cast<ScanState *>(base + SCR_STATE);
+    b.small = reinterpret_cast<uint64_t *>(base + SCR_SMALL);
+    size_t off = SCR_DYN;
+    b.cw = reinterpret_cast<CompressWord *>(base + off);
+    off += align256(cw_bytes);
+    b.clsk = reinterpret_cast<uint64_t *>(base + off);
+    off += align256(clsk_bytes);
+    b.pend = reinterpret_cast<Pend *>(base + off);
+    off += align256(sizeof(Pend) * pcap);
+    b.cand[0] = reinterpret_cast<Cand *>(base + off);
+    off += align256(sizeof(Cand) * cap);
+    b.cand[1] = reinterpret_cast<Cand *>(base + off);
     return GEPI_OK;
 }
 
@@ -224,7 +326,7 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 
 int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case,
              const gepi_params *par, gepi_result *tops, int64_t *n_tops,
-             gepi_scan_stats *stats) {
+             gepi_scan_stats *stats, bool noalloc) {
     auto t_start = std::chrono::steady_clock::now();
     int rc;
     if ((rc = validate_params(par)) != GEPI_OK) return rc;
@@ -267,6 +369,9 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     }
     int path = par->count_path;
     if (path == 0) path = 3;
+    if (noalloc && path != 3)
+        return gepi_fail(GEPI_ERR_ARG, "gepi_scan_noalloc runs the fp4 count path only (path %d "
+                                       "asked); use gepi_scan", path);
     if (path == 2 && cls0_high < 0)
         return gepi_fail(GEPI_ERR_LIMIT,
                          "MFMA count path needs a class with at most %d samples (have %d and %d)",
@@ -296,31 +401,20 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const int64_t pairs_per_tile = path == 3 ? (int64_t)TM * TB4 : (int64_t)tv * tv;
     const int32_t n_ks = path == 3 ? wused / 4 : wused / 2;
 
-    if ((rc = ensure_tables(p->tabs, n0 + n1 + 1, p->stream)) != GEPI_OK) return rc;
-    if (path != 3 &&
-        (rc = grow(&p->d_cls, &p->cls_bytes, (size_t)n_var_pad * 3 * wc * 4)) != GEPI_OK)
-        return rc;
     /* int8: n_ks 64-sample words per plane row; fp4: n_u 64-byte super-step
      * rows, then the rows' fourth-K-step set bits ([2][n_var_pad * 3] int32) */
     const int32_t n_u = f4_super(w0 / 4) + f4_super(w1 / 4);
     const size_t f4_lay = (size_t)std::max(n_u, 1) * n_var_pad * 3 * 64;
-    if (path >= 2 &&
-        (rc = grow(&p->d_clsk, &p->clsk_bytes,
-                   path == 3 ? f4_lay + (size_t)2 * n_var_pad * 3 * sizeof(int32_t)
-                             : (size_t)std::max(n_ks, 1) * n_var_pad * 3 * 8)) != GEPI_OK)
-        return rc;
-    int32_t *d_pc3 = path == 3 ? reinterpret_cast<int32_t *>(
-                                     reinterpret_cast<char *>(p->d_clsk) + f4_lay)
-                               : nullptr;
-    if ((rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK) return rc;
+    const size_t cls_bytes = path != 3 ? (size_t)n_var_pad * 3 * wc * 4 : 0;
+    const size_t clsk_bytes =
+        path == 3 ? f4_lay + (size_t)2 * n_var_pad * 3 * sizeof(int32_t)
+                  : path == 2 ? (size_t)std::max(n_ks, 1) * n_var_pad * 3 * 8 : 0;
     /* fp4 path: per 32-sample word of the panel, the compression of each
      * class (mask bits at or beyond n_samples are dropped) */
     std::vector<CompressWord> cw;
     if (path == 3) {
         int32_t c0n, c1n;
         build_compress_words(mask_ctrl, mask_case, p->n_samples, p->w64, cw, c0n, c1n);
-        if ((rc = grow(&p->d_cw, &p->cw_bytes, cw.size() * sizeof(CompressWord))) != GEPI_OK)
-            return rc;
     }
     /* Candidate buffers: between bands at most k records are kept, and a
      * one-tile band admits at most TILE_EVALS_MAX, so cap >= k +
@@ -340,47 +434,92 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         pcap = std::max<unsigned long long>((unsigned long long)std::max(atoll(env), 0ll),
                                             (unsigned long long)TILE_EVALS_MAX + 1024);
     }
-    if (cap > p->buf_cap) {
-        hipFree(p->d_cand[0]);
-        hipFree(p->d_cand[1]);
-        p->d_cand[0] = p->d_cand[1] = nullptr;
-        p->buf_cap = 0;
-        HIP_TRY(hipMalloc(&p->d_cand[0], sizeof(Cand) * cap));
-        HIP_TRY(hipMalloc(&p->d_cand[1], sizeof(Cand) * cap));
-        p->buf_cap = cap;
+    ScanBufs b;
+    if (noalloc) {
+        /* the smallest lists that always complete: more, smaller bands */
+        cap = pcap = cap_min;
+        if ((rc = bind_scratch(p, b, n0 + n1 + 1, cw.size() * sizeof(CompressWord), clsk_bytes,
+                               cap, pcap)) != GEPI_OK)
+            return rc;
+    } else {
+        /* device bytes allocated by this call: buffers that grow */
+        int64_t &alloc = st_local.bytes_allocated;
+        if (n0 + n1 + 1 > p->tabs.tmax) alloc += (int64_t)3 * (n0 + n1 + 2) * sizeof(double);
+        if ((rc = ensure_tables(p->tabs, n0 + n1 + 1, p->stream)) != GEPI_OK) return rc;
+        auto grown = [&alloc](const void *ptr, size_t have, size_t need) {
+            if (need > have || !ptr) alloc += (int64_t)need;
+        };
+        if (cls_bytes) grown(p->d_cls, p->cls_bytes, cls_bytes);
+        if (cls_bytes && (rc = grow(&p->d_cls, &p->cls_bytes, cls_bytes)) != GEPI_OK) return rc;
+        if (clsk_bytes) grown(p->d_clsk, p->clsk_bytes, clsk_bytes);
+        if (clsk_bytes && (rc = grow(&p->d_clsk, &p->clsk_bytes, clsk_bytes)) != GEPI_OK)
+            return rc;
+        if (path != 3) grown(p->d_perm, p->perm_bytes, perm.size() * 4);
+        if (path != 3 && (rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK)
+            return rc;
+        if (path == 3) grown(p->d_cw, p->cw_bytes, cw.size() * sizeof(CompressWord));
+        if (path == 3 &&
+            (rc = grow(&p->d_cw, &p->cw_bytes, cw.size() * sizeof(CompressWord))) != GEPI_OK)
+            return rc;
+        if (cap > p->buf_cap) alloc += (int64_t)(2 * sizeof(Cand) * cap);
+        if (pcap > p->pend_cap) alloc += (int64_t)(sizeof(Pend) * pcap);
+        if (cap > p->buf_cap) {
+            hipFree(p->d_cand[0]);
+            hipFree(p->d_cand[1]);
+            p->d_cand[0] = p->d_cand[1] = nullptr;
+            p->buf_cap = 0;
+            HIP_TRY(hipMalloc(&p->d_cand[0], sizeof(Cand) * cap));
+            HIP_TRY(hipMalloc(&p->d_cand[1], sizeof(Cand) * cap));
+            p->buf_cap = cap;
+        }
+        if (pcap > p->pend_cap) {
+            hipFree(p->d_pend);
+            p->d_pend = nullptr;
+            p->pend_cap = 0;
+            HIP_TRY(hipMalloc(&p->d_pend, sizeof(Pend) * pcap));
+            p->pend_cap = pcap;
+        }
+        b.cls = p->d_cls;
+        b.clsk = p->d_clsk;
+        b.perm = p->d_perm;
+        b.cw = p->d_cw;
+        b.cand[0] = p->d_cand[0];
+        b.cand[1] = p->d_cand[1];
+        b.pend = p->d_pend;
+        b.state = p->d_state;
+        b.small = p->d_small;
+        b.lhi = p->tabs.lhi;
+        b.llo = p->tabs.llo;
+        b.lfact = p->tabs.lfact;
     }
-    if (pcap > p->pend_cap) {
-        hipFree(p->d_pend);
-        p->d_pend = nullptr;
-        p->pend_cap = 0;
-        HIP_TRY(hipMalloc(&p->d_pend, sizeof(Pend) * pcap));
-        p->pend_cap = pcap;
-    }
+    int32_t *d_pc3 = path == 3 ? reinterpret_cast<int32_t *>(
+                                     reinterpret_cast<char *>(b.clsk) + f4_lay)
+                               : nullptr;
 
     if (path == 3)
-        HIP_TRY(hipMemcpyAsync(p->d_cw, cw.data(), cw.size() * sizeof(CompressWord),
+        HIP_TRY(hipMemcpyAsync(b.cw, cw.data(), cw.size() * sizeof(CompressWord),
                                hipMemcpyHostToDevice, p->stream));
     else
-        HIP_TRY(hipMemcpyAsync(p->d_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice,
+        HIP_TRY(hipMemcpyAsync(b.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice,
                                p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
     if (path == 3) {
         const int64_t rows = n_var_pad * 3;
         if (n_ks > 0) {
             k_class_f4<4><<<(unsigned)((rows + 255) / 256), 256, 0, p->stream>>>(
-                p->d_nat, (int32_t)n_var, p->w64, p->d_cw, rows, f4_super(w0 / 4),
-                reinterpret_cast<uint4 *>(p->d_clsk), d_pc3);
+                p->d_nat, (int32_t)n_var, p->w64, b.cw, rows, f4_super(w0 / 4),
+                reinterpret_cast<uint4 *>(b.clsk), d_pc3);
             HIP_TRY(hipGetLastError());
         }
     } else {
         size_t lds = (size_t)3 * (p->w64 > 0 ? p->w64 : 1) * 8;
         k_gather_class<<<(unsigned)n_var_pad, 256, lds, p->stream>>>(
-            p->d_nat, (int32_t)n_var, p->w64, p->d_perm, wc, wused, p->d_cls);
+            p->d_nat, (int32_t)n_var, p->w64, b.perm, wc, wused, b.cls);
         HIP_TRY(hipGetLastError());
         if (path == 2 && n_ks > 0) {
             dim3 grid((unsigned)(n_var_pad / 32), (unsigned)((n_ks + 31) / 32));
-            k_transpose_k<<<grid, 256, 0, p->stream>>>(p->d_cls, (int32_t)n_var_pad, wc, n_ks,
-                                                       p->d_clsk);
+            k_transpose_k<<<grid, 256, 0, p->stream>>>(b.cls, (int32_t)n_var_pad, wc, n_ks,
+                                                       b.clsk);
             HIP_TRY(hipGetLastError());
         }
 
@@ -391,7 +530,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     memset(&init, 0, sizeof init);
     init.thr_score = INFINITY;
     init.thr_idx = ~0ull;
-    HIP_TRY(hipMemcpyAsync(p->d_state, &init, sizeof init, hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(hipMemcpyAsync(b.state, &init, sizeof init, hipMemcpyHostToDevice, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
     {
         float ms = 0;
@@ -402,8 +541,8 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     int cur = 0; /* active candidate buffer */
     CountArgs ca;
     memset(&ca, 0, sizeof ca);
-    ca.planes = p->d_cls;
-    ca.planes_k = p->d_clsk;
+    ca.planes = b.cls;
+    ca.planes_k = b.clsk;
     ca.pc3 = d_pc3;
     ca.n_var = (int32_t)n_var;
     ca.n_var_pad = (int32_t)n_var_pad;
@@ -416,17 +555,17 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ca.f32 = f32_counts(n0, n1) ? 1 : 0;
     ca.n0 = n0;
     ca.n1 = n1;
-    ca.st = p->d_state;
-    ca.pend = p->d_pend;
+    ca.st = b.state;
+    ca.pend = b.pend;
     ca.pcap = pcap;
-    ca.lfact = p->tabs.lfact;
+    ca.lfact = b.lfact;
     ExactArgs ea;
-    ea.st = p->d_state;
-    ea.pend = p->d_pend;
+    ea.st = b.state;
+    ea.pend = b.pend;
     ea.pcap = pcap;
     ea.cap = cap;
-    ea.lhi = p->tabs.lhi;
-    ea.llo = p->tabs.llo;
+    ea.lhi = b.lhi;
+    ea.llo = b.llo;
 
     const int64_t evals_per_tile = pairs_per_tile * 20;
     /* Bands grow 4x per band up to 8e9 evaluations (195,000 fp4 tiles): each
@@ -456,8 +595,8 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         const int64_t t1 = std::min<int64_t>(n_tiles, t0 + band);
         ca.tile_begin = t0;
         ca.n_tiles = (int32_t)(t1 - t0);
-        ea.cand = p->d_cand[cur];
-        k_reset_pending<<<1, 1, 0, p->stream>>>(p->d_state);
+        ea.cand = b.cand[cur];
+        k_reset_pending<<<1, 1, 0, p->stream>>>(b.state);
         HIP_TRY(hipEventRecord(p->ev0, p->stream));
         const unsigned grid = (unsigned)(t1 - t0);
         if (path == 3) {
@@ -489,7 +628,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             k_exact<GEPI_FFUNC_FISHERPV><<<1024, 256, 0, p->stream>>>(ea);
         HIP_TRY(hipGetLastError());
         ScanState hs;
-        if ((rc = read_state(p, &hs)) != GEPI_OK) return rc;
+        if ((rc = read_state(p, b, &hs)) != GEPI_OK) return rc;
         float ms = 0;
         hipEventElapsedTime(&ms, p->ev0, p->ev1);
         ms_count += ms;
@@ -511,15 +650,15 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             const unsigned long long kept = std::min<unsigned long long>(hs.count, cap);
             hs.count = kept;
             hs.overflow = 0;
-            HIP_TRY(hipMemcpyAsync(p->d_state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
-            launch_select(p, p->d_cand[cur], (uint32_t)k, kept);
-            k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
+            HIP_TRY(hipMemcpyAsync(b.state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
+            launch_select(p, b, b.cand[cur], (uint32_t)k, kept);
+            k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
                                                    t0, t1, tv, path == 3 ? (int32_t)n_a4 : 0);
-            k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
+            k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
             HIP_TRY(hipGetLastError());
             cur ^= 1;
             ScanState h2;
-            if ((rc = read_state(p, &h2)) != GEPI_OK) return rc;
+            if ((rc = read_state(p, b, &h2)) != GEPI_OK) return rc;
             count = h2.count;
             exact_total -= hs.pend_count;
             band = std::max<int64_t>(1, (t1 - t0) / 4);
@@ -527,10 +666,10 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             continue;
         }
         if (hs.count > (unsigned long long)k) {
-            launch_select(p, p->d_cand[cur], (uint32_t)k, hs.count);
-            k_compact<<<4096, 256, 0, p->stream>>>(p->d_cand[cur], p->d_state, p->d_cand[cur ^ 1],
+            launch_select(p, b, b.cand[cur], (uint32_t)k, hs.count);
+            k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
                                                    0, 0, tv, path == 3 ? (int32_t)n_a4 : 0);
-            k_finalize<<<1, 1, 0, p->stream>>>(p->d_state, par->ffun);
+            k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
             HIP_TRY(hipGetLastError());
             cur ^= 1;
             count = (unsigned long long)k;
@@ -551,7 +690,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     auto ts = std::chrono::steady_clock::now();
     std::vector<Cand> host((size_t)count);
     if (count > 0)
-        HIP_TRY(hipMemcpyAsync(host.data(), p->d_cand[cur], sizeof(Cand) * count,
+        HIP_TRY(hipMemcpyAsync(host.data(), b.cand[cur], sizeof(Cand) * count,
                                hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
     const int64_t n = candidates_to_results(host, k, tops);
@@ -839,7 +978,14 @@ extern "C" int gepi_scan(gepi_panel *panel, const uint64_t *mask_ctrl, const uin
                          const gepi_params *par, gepi_result *tops, int64_t *n_tops,
                          gepi_scan_stats *stats) {
     if (!panel) return gepi_fail(GEPI_ERR_ARG, "null panel");
-    return run_scan(panel, mask_ctrl, mask_case, par, tops, n_tops, stats);
+    return run_scan(panel, mask_ctrl, mask_case, par, tops, n_tops, stats, false);
+}
+
+extern "C" int gepi_scan_noalloc(gepi_panel *panel, const uint64_t *mask_ctrl,
+                                 const uint64_t *mask_case, const gepi_params *par,
+                                 gepi_result *tops, int64_t *n_tops, gepi_scan_stats *stats) {
+    if (!panel) return gepi_fail(GEPI_ERR_ARG, "null panel");
+    return run_scan(panel, mask_ctrl, mask_case, par, tops, n_tops, stats, true);
 }
 
 extern "C" int gepi_process_gpu_1(const uint64_t *bit_data, const uint64_t *valid_data,
@@ -852,7 +998,7 @@ extern "C" int gepi_process_gpu_1(const uint64_t *bit_data, const uint64_t *vali
     if ((rc = gepi_panel_create(bit_data, valid_data, n_rows, n_samples, par->gpu_num, &p)) !=
         GEPI_OK)
         return rc;
-    rc = run_scan(p, mask_ctrl, mask_case, par, tops, n_tops, stats);
+    rc = run_scan(p, mask_ctrl, mask_case, par, tops, n_tops, stats, false);
     panel_free(p);
     return rc;
 }

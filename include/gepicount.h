@@ -91,6 +91,8 @@ typedef struct gepi_scan_stats {
     int32_t n_class0, n_class1;
     int32_t pend_overflows;  /* bands redone: filter survivors overflowed      */
     int32_t cand_overflows;  /* bands redone: admitted candidates overflowed   */
+    int64_t bytes_allocated; /* device memory by which this call grew the
+                                scan workspace; always 0 for gepi_scan_noalloc */
 } gepi_scan_stats;
 
 /* Defaults of src/gEpiCount.cpp:34-39 and :100-115. */
@@ -130,6 +132,21 @@ int gepi_panel_destroy(gepi_panel *panel);
 int gepi_scan(gepi_panel *panel, const uint64_t *mask_ctrl, const uint64_t *mask_case,
               const gepi_params *par, gepi_result *tops, int64_t *n_tops,
               gepi_scan_stats *stats);
+/* gepi_scan without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, allocation-free serving loops).  Every buffer a
+ * scan needs (score tables, scan state, class layout, filter survivors,
+ * candidate lists) lives in one fixed array of GEPI_NOALLOC_SCRATCH_MIB MiB
+ * per GPU, held until the process ends.  Fp4 count path only (count_path 0
+ * or 3); a scan whose buffers do not fit fails with GEPI_ERR_LIMIT (with
+ * top-k up to 100,000: about 60,000 variants at 2,048 samples, 15,000 at
+ * 8,192).
+ * Results equal gepi_scan's.  Calls on one GPU share the array: calls that
+ * overlap on one GPU (from several threads) race and must not be made.      */
+#define GEPI_NOALLOC_SCRATCH_MIB 64
+int gepi_scan_noalloc(gepi_panel *panel, const uint64_t *mask_ctrl, const uint64_t *mask_case,
+                      const gepi_params *par, gepi_result *tops, int64_t *n_tops,
+                      gepi_scan_stats *stats);
 /* Reads the panel's planes back in reader layout (testing aid). */
 int gepi_panel_download(gepi_panel *panel, uint64_t *bit_data, uint64_t *valid_data);
 /* Missing-call QA on the device: the counts of checkQuality
