@@ -147,203 +147,10 @@ __global__ void __launch_bounds__(256) k_exact_batch(const ExactDesc *__restrict
 }
 
 /* What the host needs after a round, per scan. */
-struct ItemReport {
-    unsigned long long raw_count;  /* candidates after k_exact, before top-k  */
-    unsigned long long count;      /* candidates kept after the round         */
-    unsigned long long pend_count; /* filter survivors of the band            */
-    uint32_t pend_overflow;        /* survivors lost: band to be redone       */
-    uint32_t cand_overflow;        /* candidates lost: band dropped, redone   */
-};
-
-struct TopkDesc {
-    ScanState *st;
-    Cand *cand;
-    unsigned long long cap;
-    int64_t drop_t0, drop_t1; /* this round's band (fp4 tile numbering)       */
-    ItemReport *report;
-    uint32_t k;
-    int32_t ffun;
-    int32_t n_a4;             /* A blocks of the scan (fp4 tile numbering)    */
-    int32_t pad;
-};
-
-constexpr int TOPK_THREADS = 1024;
-
-__device__ __forceinline__ bool key_le(uint64_t ah, uint64_t al, uint64_t bh, uint64_t bl) {
-    return ah < bh || (ah == bh && al <= bl);
-}
-
-/* Block-wide exclusive prefix of one flag per thread (1024 threads). */
-__device__ __forceinline__ uint32_t block_prefix(bool flag, uint32_t *wsum, uint32_t &total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t b = __ballot(flag);
-    const uint32_t below = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
-    if (lane == 0) wsum[wave] = (uint32_t)__popcll(b);
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (int w = 0; w < TOPK_THREADS / 64; w++) {
-        const uint32_t c = wsum[w];
-        before += w < wave ? c : 0u;
-        all += c;
-    }
-    total = all;
-    return before + below;
-}
-
-/* Top-k maintenance of one scan by one workgroup: after a candidate
- * overflow, the k-th key among the records kept becomes the threshold and the
- * band's records are dropped (the band is redone); otherwise, with more than
- * k candidates, exactly the k smallest keys are kept.  Resets the pending
- * list for the next round.                                                 */
+/* k_topk_wg: the top-k maintenance of every scan of a round, one workgroup
+ * per descriptor (topk_wg_body in gepi_topk.h). */
 __global__ void __launch_bounds__(TOPK_THREADS) k_topk_wg(const TopkDesc *__restrict__ dd) {
-    const TopkDesc d = dd[blockIdx.x];
-    ScanState *st = d.st;
-    if (st == nullptr) return; /* empty descriptor slot (the whole workgroup) */
-    __shared__ uint32_t hist[256];
-    __shared__ uint64_t kh[SEL_SMALL], kl[SEL_SMALL];
-    __shared__ uint32_t wsum[TOPK_THREADS / 64];
-    __shared__ uint64_t s_hi, s_lo;
-    __shared__ uint32_t s_bits, s_rank, s_done, s_gather;
-    __shared__ unsigned long long s_wptr;
-    const int t = threadIdx.x;
-
-    const unsigned long long raw = st->count;
-    const uint32_t pend_ovf = st->pend_overflow, cand_ovf = st->overflow;
-    const unsigned long long pend_n = st->pend_count;
-    __syncthreads(); /* every thread has read the state before it changes */
-    const unsigned long long n = raw < d.cap ? raw : d.cap;
-    const bool work = !pend_ovf && (cand_ovf || n > d.k);
-    if (work) {
-        if (t == 0) {
-            s_hi = 0;
-            s_lo = 0;
-            s_bits = 0;
-            s_rank = d.k;
-            s_done = 0;
-        }
-        /* radix select of the k-th smallest key, 8-bit digits MSB first */
-        while (true) {
-            if (t < 256) hist[t] = 0;
-            __syncthreads();
-            SelPrefix pre;
-            pre.key_hi = s_hi;
-            pre.key_lo = s_lo;
-            pre.bits_done = s_bits;
-            for (unsigned long long i = t; i < n; i += TOPK_THREADS) {
-                const uint64_t h = order_bits(d.cand[i].score), l = d.cand[i].idx;
-                if (key_match(h, l, pre)) atomicAdd(&hist[key_digit(h, l, pre.bits_done)], 1u);
-            }
-            __syncthreads();
-            if (t == 0) {
-                uint32_t cum = 0, dg = 0;
-                for (dg = 0; dg < 255; dg++) {
-                    if (cum + hist[dg] >= s_rank) break;
-                    cum += hist[dg];
-                }
-                const uint32_t b = s_bits;
-                if (b < 64)
-                    s_hi |= (uint64_t)dg << (56 - b);
-                else
-                    s_lo |= (uint64_t)dg << (120 - b);
-                s_rank -= cum;
-                s_bits = b + 8;
-                s_done = s_bits == 128 ? 1u : (hist[dg] <= (uint32_t)SEL_SMALL ? 2u : 0u);
-                s_gather = 0;
-            }
-            __syncthreads();
-            if (s_done) break;
-        }
-        if (s_done == 2) {
-            /* the remaining matching keys fit in LDS: gather and sort them */
-            SelPrefix pre;
-            pre.key_hi = s_hi;
-            pre.key_lo = s_lo;
-            pre.bits_done = s_bits;
-            for (unsigned long long i = t; i < n; i += TOPK_THREADS) {
-                const uint64_t h = order_bits(d.cand[i].score), l = d.cand[i].idx;
-                if (key_match(h, l, pre)) {
-                    const uint32_t p = atomicAdd(&s_gather, 1u);
-                    if (p < (uint32_t)SEL_SMALL) {
-                        kh[p] = h;
-                        kl[p] = l;
-                    }
-                }
-            }
-            __syncthreads();
-            const uint32_t g = s_gather < (uint32_t)SEL_SMALL ? s_gather : (uint32_t)SEL_SMALL;
-            if (t >= (int)g) {
-                kh[t] = ~0ull;
-                kl[t] = ~0ull;
-            }
-            __syncthreads();
-            for (int size = 2; size <= SEL_SMALL; size <<= 1) {
-                for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                    const int partner = t ^ stride;
-                    if (partner > t) {
-                        const bool up = (t & size) == 0;
-                        const bool gt = kh[t] > kh[partner] || (kh[t] == kh[partner] && kl[t] > kl[partner]);
-                        if (gt == up) {
-                            const uint64_t a = kh[t], b = kl[t];
-                            kh[t] = kh[partner];
-                            kl[t] = kl[partner];
-                            kh[partner] = a;
-                            kl[partner] = b;
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
-            if (t == 0) {
-                s_hi = kh[s_rank - 1];
-                s_lo = kl[s_rank - 1];
-            }
-            __syncthreads();
-        }
-        /* in-place compaction: a chunk is read before any of it is written,
-         * and writes never pass the read position */
-        const uint64_t khi = s_hi, klo = s_lo;
-        if (t == 0) s_wptr = 0;
-        __syncthreads();
-        for (unsigned long long base = 0; base < n; base += TOPK_THREADS) {
-            const unsigned long long i = base + t;
-            Cand c;
-            bool keep = false;
-            if (i < n) {
-                c = d.cand[i];
-                keep = key_le(order_bits(c.score), c.idx, khi, klo);
-                if (keep && cand_ovf) {
-                    const int64_t a = (int64_t)(c.idx >> 32) >> 1;
-                    const int64_t b = (int64_t)((c.idx >> 3) & 0x1FFFFFFFull) >> 1;
-                    const int64_t tl = tile_index(a, b, TM, d.n_a4);
-                    keep = tl < d.drop_t0 || tl >= d.drop_t1;
-                }
-            }
-            uint32_t total;
-            const uint32_t off = block_prefix(keep, wsum, total);
-            const unsigned long long w0 = s_wptr;
-            __syncthreads();
-            if (keep) d.cand[w0 + off] = c;
-            if (t == 0) s_wptr = w0 + total;
-            __syncthreads();
-        }
-        if (t == 0) {
-            st->count = s_wptr;
-            st->count2 = 0;
-            st->overflow = 0;
-            adopt_threshold(st, khi, klo, d.ffun);
-        }
-    }
-    if (t == 0) {
-        ItemReport r;
-        r.raw_count = raw;
-        r.count = work ? s_wptr : (pend_ovf ? raw : n);
-        r.pend_count = pend_n;
-        r.pend_overflow = pend_ovf;
-        r.cand_overflow = pend_ovf ? 0u : cand_ovf;
-        *d.report = r;
-        st->pend_count = 0;
-        st->pend_overflow = 0;
-    }
+    topk_wg_body(dd[blockIdx.x]);
 }
 
 } // namespace gepi

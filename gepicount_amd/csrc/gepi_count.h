@@ -356,6 +356,12 @@ __device__ __forceinline__ uint32_t f4_ones_vgpr() {
     return ones;
 }
 
+/* Word 3 of a gfx9 buffer resource: 32-bit data format, raw access. */
+constexpr int F4_RSRC_WORD3 = 0x00020000;
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 as_u4(u32x4 v) { return make_uint4(v[0], v[1], v[2], v[3]); }
+
 /* Block scale (E8M0) of K step k: 2^(1-k) for k < 3, 1 for the sign step. */
 __device__ __forceinline__ constexpr int f4_scale(int k) { return k < 3 ? 128 - k : 127; }
 
@@ -399,14 +405,15 @@ template <int FFUN, bool F32>
 __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) {
     __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_UNITS];
     __shared__ PendStage stage;
-    __shared__ PairList pairs;
     __shared__ F4Corr corr;
     /* Fisher flags a variant pair in about 13% of lanes: those pairs are
      * parked and finished one per thread after the tile (entropy's few are
-     * finished in the lane) */
+     * finished in the lane, and its kernels hold no parking space) */
     constexpr bool PARK = FFUN == GEPI_FFUNC_FISHERPV;
+    __shared__ std::conditional_t<PARK, PairList, uint32_t> park;
+    PairList *pairs = PARK ? reinterpret_cast<PairList *>(&park) : nullptr;
     stage_init(stage);
-    if (threadIdx.x == 0) pairs.n = 0;
+    if (PARK && threadIdx.x == 0) pairs->n = 0;
     int32_t ta, tb;
     tile_of_f4(tile, g.n_var_pad / TM, ta, tb);
 
@@ -424,22 +431,32 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
      * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
      * contiguous in the layout; thread t stores LDS units t + 256j of the A
      * rows (j < 3) and of the B rows (t + 256j < 384), see f4_unit.        */
-    const uint4 *base = reinterpret_cast<const uint4 *>(g.planes_k);
-    const int64_t ustride = (int64_t)g.n_var_pad * 3 * 4; /* uint4 per super-step */
-    const int si = f4_src(tid);
-    const uint4 *gA = base + (int64_t)ta * (F4_RA * 4) + si;
-    const uint4 *gB = base + (int64_t)tb * (TB4 * 3 * 4) + si;
+    /* the tile's rows start at workgroup-uniform addresses (scalar
+     * registers); a thread adds only its own 32-bit byte offset, so the
+     * loads take the scalar-base form with no 64-bit address arithmetic    */
+    const char *lay = reinterpret_cast<const char *>(g.planes_k);
+    const int64_t ustride_b = (int64_t)g.n_var_pad * 3 * 64; /* bytes per super-step */
+    const char *tA = lay + (int64_t)ta * (F4_RA * 64);
+    const char *tB = lay + (int64_t)tb * (TB4 * 3 * 64);
+    const uint32_t so = (uint32_t)f4_src(tid) * 16u;
     const bool hasB1 = tid < 128;
     const int dA = tid, dB = F4_RA * 4 + tid;
     constexpr int D64 = 256; /* 64 rows further down */
     uint4 pa0, pa1, pa2, pb0, pb1 = make_uint4(0u, 0u, 0u, 0u);
     auto fetch = [&](int32_t u) {
-        const int64_t o = (int64_t)u * ustride;
-        pa0 = gA[o];
-        pa1 = gA[o + 256];
-        pa2 = gA[o + 512];
-        pb0 = gB[o];
-        if (hasB1) pb1 = gB[o + 256];
+        /* buffer loads: the super-step's A and B rows as two scalar buffer
+         * resources (12,288 and 6,144 bytes), each thread adding its own
+         * 32-bit offset; no vector address arithmetic per super-step      */
+        const int64_t o = (int64_t)u * ustride_b;
+        const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char *>(tA + o), (short)0, F4_RA * 64, F4_RSRC_WORD3);
+        const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char *>(tB + o), (short)0, TB4 * 3 * 64, F4_RSRC_WORD3);
+        pa0 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so, 0, 0));
+        pa1 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so + 4096, 0, 0));
+        pa2 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so + 8192, 0, 0));
+        pb0 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(rb, (int)so, 0, 0));
+        if (hasB1) pb1 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(rb, (int)so + 4096, 0, 0));
     };
     auto put = [&](uint4 *W) {
         W[dA] = pa0;
@@ -543,7 +560,7 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 #pragma unroll
                         for (int q = 0; q < 3; q++)
                             c[k][p][q] = k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r];
-                filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, c, &pairs);
+                filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, c, pairs);
             } else {
                 int32_t c[2][3][3];
 #pragma unroll
@@ -553,13 +570,13 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
 #pragma unroll
                         for (int q = 0; q < 3; q++)
                             c[k][p][q] = (int32_t)(k == 0 ? acc0[i][p][q][r] : acc1[i][p][q][r]);
-                filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, c, &pairs);
+                filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, c, pairs);
             }
         }
     }
-    if (PARK) {
+    if constexpr (PARK) {
         __syncthreads();
-        pair_drain<FFUN>(g, stage, fc, pairs);
+        pair_drain<FFUN>(g, stage, fc, *pairs);
     }
     stage_flush(g, stage);
 }

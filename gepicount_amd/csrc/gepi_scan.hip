@@ -320,6 +320,32 @@ int bind_scratch(gepi_panel *p, ScanBufs &b, int32_t tmax, size_t cw_bytes, size
     return GEPI_OK;
 }
 
+/* Top-k maintenance by one workgroup (k_topk_one) when the candidates are
+ * few enough for one CU to pass over them several times quickly; the
+ * multi-workgroup select chain otherwise, and on the paths whose tile
+ * numbering k_topk_one does not know (it drops overflowed bands by fp4
+ * tiles).                                                                  */
+constexpr unsigned long long WG_SELECT_MAX = 1ull << 18;
+inline bool wg_select(int path, unsigned long long count) {
+    return path == 3 && count <= WG_SELECT_MAX;
+}
+
+TopkDesc topk_desc(const ScanBufs &b, int cur, unsigned long long cap, int64_t k,
+                   const gepi_params *par, int64_t t0, int64_t t1, int64_t n_a4) {
+    TopkDesc d;
+    memset(&d, 0, sizeof d);
+    d.st = b.state;
+    d.cand = b.cand[cur];
+    d.cap = cap;
+    d.drop_t0 = t0;
+    d.drop_t1 = t1;
+    d.report = nullptr;
+    d.k = (uint32_t)k;
+    d.ffun = par->ffun;
+    d.n_a4 = (int32_t)n_a4;
+    return d;
+}
+
 double ms_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
@@ -648,15 +674,21 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
              * records and redo the band                                     */
             st_local.cand_overflows++;
             const unsigned long long kept = std::min<unsigned long long>(hs.count, cap);
-            hs.count = kept;
-            hs.overflow = 0;
-            HIP_TRY(hipMemcpyAsync(b.state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
-            launch_select(p, b, b.cand[cur], (uint32_t)k, kept);
-            k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
-                                                   t0, t1, tv, path == 3 ? (int32_t)n_a4 : 0);
-            k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
+            if (wg_select(path, kept)) {
+                /* reads the overflow flag, clamps the count to cap and drops
+                 * the band's records itself */
+                k_topk_one<<<1, TOPK_THREADS, 0, p->stream>>>(topk_desc(b, cur, cap, k, par, t0, t1, n_a4));
+            } else {
+                hs.count = kept;
+                hs.overflow = 0;
+                HIP_TRY(hipMemcpyAsync(b.state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
+                launch_select(p, b, b.cand[cur], (uint32_t)k, kept);
+                k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
+                                                       t0, t1, tv, path == 3 ? (int32_t)n_a4 : 0);
+                k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
+                cur ^= 1;
+            }
             HIP_TRY(hipGetLastError());
-            cur ^= 1;
             ScanState h2;
             if ((rc = read_state(p, b, &h2)) != GEPI_OK) return rc;
             count = h2.count;
@@ -666,12 +698,16 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             continue;
         }
         if (hs.count > (unsigned long long)k) {
-            launch_select(p, b, b.cand[cur], (uint32_t)k, hs.count);
-            k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
-                                                   0, 0, tv, path == 3 ? (int32_t)n_a4 : 0);
-            k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
+            if (wg_select(path, hs.count)) {
+                k_topk_one<<<1, TOPK_THREADS, 0, p->stream>>>(topk_desc(b, cur, cap, k, par, 0, 0, n_a4));
+            } else {
+                launch_select(p, b, b.cand[cur], (uint32_t)k, hs.count);
+                k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
+                                                       0, 0, tv, path == 3 ? (int32_t)n_a4 : 0);
+                k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
+                cur ^= 1;
+            }
             HIP_TRY(hipGetLastError());
-            cur ^= 1;
             count = (unsigned long long)k;
         } else {
             count = hs.count;
