@@ -196,7 +196,8 @@ def run_mix(args):
         "speedup_vs_sequential": statistics.mean(seq_t) / statistics.mean(bat_t),
         "results_identical_to_sequential": identical,
         "rounds": bstats[-1]["rounds"], "graph_rounds": bstats[-1]["graph_rounds"],
-        "autotune": {"s": tune_s, "classes": [{k: c[k] for k in ("max_ksteps", "wg_per_cu", "band_evals")}
+        "autotune": {"s": tune_s, "classes": [{k: c[k] for k in ("max_ksteps", "tile", "wg_per_cu",
+                                                                 "band_evals", "ms_best", "ms_default")}
                                               for c in tune]} if args.autotune else None,
         "scan_latency_in_batch_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1,
                                                                          int(0.99 * len(lat)))]},
@@ -280,18 +281,28 @@ def run_serve(args):
     st = srv.stats()
     lat = sorted(r[1]["ms_scan"] for r in results)
     evals = sum(r[1]["evals"] for r in results)
+    # saturated: every request submitted at once (the server's own limit)
+    t0 = time.perf_counter()
+    sat_tickets = [srv.submit(pid, m0, m1, topmost=100, ffun=args.ffun)
+                   for pid, m0, m1, _, _, _ in reqs]
+    sat_results = [srv.wait(t) for t in sat_tickets]
+    sat_wall = time.perf_counter() - t0
     srv.close()
-    # every response (scans of the mix and re-scans of the resident panel)
-    # against the same scan run alone
-    identical, checked = True, 0
+    # one at a time: the same requests as single scans of resident panels,
+    # one after another; every response (scans of the mix and re-scans of the
+    # resident panel) must equal the served ones
     handles = {}
-    for i, (pid, m0, m1, _, kind, p) in enumerate(reqs):
+    for pid, m0, m1, _, kind, p in reqs:
         if pid not in handles:
             src = big if kind == "rescan" else p
             handles[pid] = ge.Panel.from_bed(src["payload"], src["n_var"], src["n_samples"])
-        alone, _ = handles[pid].scan(m0, m1, topmost=100, ffun=args.ffun, count_path=3)
-        identical &= bool(np.array_equal(alone, results[i][0]))
-        checked += 1
+    t0 = time.perf_counter()
+    alone = [handles[pid].scan(m0, m1, topmost=100, ffun=args.ffun, count_path=3)[0]
+             for pid, m0, m1, _, _, _ in reqs]
+    seq_wall = time.perf_counter() - t0
+    identical = all(np.array_equal(a, r[0]) and np.array_equal(a, q[0])
+                    for a, r, q in zip(alone, results, sat_results))
+    checked = len(alone)
     for h in handles.values():
         h.close()
     line = {
@@ -310,6 +321,11 @@ def run_serve(args):
         "rounds": st["rounds"] - base["rounds"], "max_active": st["max_active"],
         "failed": st["failed"], "responses_identical_to_alone": identical,
         "responses_checked": checked,
+        "saturated": {"evals_per_s": evals / sat_wall, "wall_s": sat_wall,
+                      "note": "all requests submitted at once"},
+        "one_at_a_time": {"evals_per_s": evals / seq_wall, "wall_s": seq_wall,
+                          "note": "the same requests as single scans, one after another"},
+        "served_vs_one_at_a_time": seq_wall / sat_wall,
         "roofline": None, "cpu_baseline": None, "gen_s": gen_s,
         "autotune_s": tune_s if args.autotune else None,
     }

@@ -86,6 +86,7 @@ class BatchStats(ctypes.Structure):
 
 class TuneClass(ctypes.Structure):
     _fields_ = [("max_ksteps", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("tile", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("band_evals", ctypes.c_double), ("ms_best", ctypes.c_double),
                 ("ms_default", ctypes.c_double)]
 
@@ -409,8 +410,9 @@ class Engine:
 
     def autotune(self, repeats=2):
         """Runs the batch autotuner (gepi_engine_autotune) and returns its
-        table: one dict per size class (max_ksteps, wg_per_cu, band_evals,
-        ms_best, ms_default), ascending max_ksteps."""
+        table: one dict per size class (max_ksteps, wg_per_cu, tile,
+        band_evals, ms_best, ms_default), ascending max_ksteps; tile 0 is the
+        64 x 32-variant count job, 1 the 32 x 32 half tile."""
         out = (TuneClass * 16)()
         n = ctypes.c_int32()
         _check(lib().gepi_engine_autotune(self._h, int(repeats), out, 16, ctypes.byref(n)))
@@ -423,6 +425,7 @@ class Engine:
         for i, c in enumerate(classes):
             arr[i].max_ksteps = int(c["max_ksteps"])
             arr[i].wg_per_cu = int(c["wg_per_cu"])
+            arr[i].tile = int(c.get("tile", 0))
             arr[i].band_evals = float(c["band_evals"])
             arr[i].ms_best = float(c.get("ms_best", 0.0))
             arr[i].ms_default = float(c.get("ms_default", 0.0))

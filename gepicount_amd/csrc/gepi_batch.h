@@ -107,17 +107,18 @@ struct F4BatchOnceSrc {
 
 /* PERSIST: workgroups take slots from the work counters (grid of a few
  * workgroups per CU); else one slot per workgroup (grid = 8 x the largest
- * XCD share). */
-template <int FFUN, bool PERSIST>
+ * XCD share).  HALF: the jobs are half tiles (32 x 32 variants; the
+ * descriptors' tile_begin and n_tiles count half tiles, two per tile). */
+template <int FFUN, bool PERSIST, bool HALF>
 __global__ void __launch_bounds__(F4_THREADS, 2)
     k_count_f4_batch(const BatchDesc *__restrict__ d, BatchMap map) {
     if (PERSIST) {
         zero_other_counters(map.wq, map.wq_parity);
         const F4BatchSrc src{d, map};
-        count_f4_run<FFUN>(src);
+        count_f4_run<FFUN, HALF>(src);
     } else {
         const F4BatchOnceSrc src{d, map};
-        count_f4_run<FFUN>(src);
+        count_f4_run<FFUN, HALF>(src);
     }
 }
 

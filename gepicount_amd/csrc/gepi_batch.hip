@@ -56,8 +56,8 @@ int Engine::init(int dev) {
     HIP_TRY(hipEventCreate(&ev0_));
     HIP_TRY(hipEventCreate(&ev1_));
     /* fp4 work counters: per objective launch, 2 sets of 8 */
-    HIP_TRY(hipMalloc(&d_wq_, sizeof(uint32_t) * 32));
-    HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
+    HIP_TRY(hipMalloc(&d_wq_, sizeof(uint32_t) * 16 * NGROUP));
+    HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 16 * NGROUP, stream));
     /* rounds as captured graphs only on request: on the MI355X a graph
      * launch costs about 0.16 ms per round more than the direct launches */
     const char *g = getenv("GEPI_ENGINE_GRAPH");
@@ -76,28 +76,39 @@ void Engine::drop_graphs() {
     gx_per_cu_ = 0;
 }
 
+void Engine::launch_count(int g, bool persist, unsigned grid, const BatchMap &m) {
+    const BatchDesc *d = d_bdesc_[g];
+    switch (g | (persist ? 4 : 0)) {
+    case 0: k_count_f4_batch<GEPI_FFUNC_ENTROPY, false, false><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    case 1: k_count_f4_batch<GEPI_FFUNC_FISHERPV, false, false><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    case 2: k_count_f4_batch<GEPI_FFUNC_ENTROPY, false, true><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    case 3: k_count_f4_batch<GEPI_FFUNC_FISHERPV, false, true><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    case 4: k_count_f4_batch<GEPI_FFUNC_ENTROPY, true, false><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    case 5: k_count_f4_batch<GEPI_FFUNC_FISHERPV, true, false><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    case 6: k_count_f4_batch<GEPI_FFUNC_ENTROPY, true, true><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    default: k_count_f4_batch<GEPI_FFUNC_FISHERPV, true, true><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
+    }
+}
+
 int Engine::capture_graph(int part, int per_cu) {
     const size_t cap = desc_cap_;
     const unsigned grid =
         (unsigned)std::max<int64_t>(8, (per_cu * (int64_t)device_cus(device)) / 8 * 8);
     /* nothing may return between the begin and the end of the capture */
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    if (part < 4) {
-        k_fill_u32<<<1, 64, 0, stream>>>(d_wq_, 32, 0u);
-        for (int f = 0; f < 2; f++) {
-            if (!(part & (1 << f))) continue;
+    if (part < NGRAPH_COUNT) {
+        k_fill_u32<<<1, 64, 0, stream>>>(d_wq_, 16 * NGROUP, 0u);
+        for (int g = 0; g < NGROUP; g++) {
+            if (!(part & (1 << g))) continue;
             /* the maps are laid out for cap scans; the slots past a round's
              * scans have empty ranges                                       */
             BatchMap m;
-            m.xprefix = d_map_[f];
-            m.xstart = d_map_[f] + 8 * (cap + 1);
+            m.xprefix = d_map_[g];
+            m.xstart = d_map_[g] + 8 * (cap + 1);
             m.n = (int32_t)cap;
-            m.wq = d_wq_ + 16 * f;
+            m.wq = d_wq_ + 16 * g;
             m.wq_parity = 0;
-            if (f == 0)
-                k_count_f4_batch<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-            else
-                k_count_f4_batch<GEPI_FFUNC_FISHERPV, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
+            launch_count(g, true, grid, m);
         }
     } else {
         k_exact_batch<<<dim3(EXACT_BLOCKS_X, (unsigned)cap), 256, 0, stream>>>(d_edesc_);
@@ -108,7 +119,8 @@ int Engine::capture_graph(int part, int per_cu) {
     const hipError_t ce = hipStreamEndCapture(stream, &graph);
     hipError_t ie = hipSuccess;
     if (le == hipSuccess && ce == hipSuccess)
-        ie = hipGraphInstantiate(part < 4 ? &gx_count_[part] : &gx_finish_, graph, nullptr, nullptr, 0);
+        ie = hipGraphInstantiate(part < NGRAPH_COUNT ? &gx_count_[part] : &gx_finish_, graph, nullptr,
+                                 nullptr, 0);
     if (graph) hipGraphDestroy(graph);
     if (le != hipSuccess || ce != hipSuccess || ie != hipSuccess) {
         const hipError_t e = le != hipSuccess ? le : ce != hipSuccess ? ce : ie;
@@ -132,7 +144,7 @@ void Engine::release() {
     for (auto &b : live_) hipFree(b.first);
     live_.clear();
     in_use_ = 0;
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < NGROUP; i++) {
         hipFree(d_bdesc_[i]);
         hipFree(d_map_[i]);
         d_bdesc_[i] = nullptr;
@@ -250,60 +262,80 @@ int Engine::autotune(int32_t repeats) {
         int64_t n_var;
         int32_t n;
     };
-    const Trial trials[4] = {{2, 16000, 256}, {8, 10000, 1024}, {32, 8000, 4096}, {INT32_MAX, 6000, 8192}};
-    const int per_cus[3] = {0, 1, 2};
+    /* trial batches large enough that a schedule's per-job costs, not the
+     * tail of a short launch, decide (tens of ms each) */
+    const Trial trials[4] = {{2, 40000, 256}, {8, 28000, 1024}, {32, 18000, 4096}, {INT32_MAX, 12000, 8192}};
+    const int per_cus[4] = {0, 1, 2, 3};
     const double bands[3] = {5e8, 2e9, 8e9};
+    const int tiles[2] = {0, 1};
     repeats = std::max(1, std::min(repeats, 16));
     std::vector<gepi_tune_class> table;
     int rc = GEPI_OK;
     tuning.clear();
-    for (int c = 0; c < 4 && rc == GEPI_OK; c++) {
-        const Trial &t = trials[c];
-        const std::vector<uint8_t> bed = synth_bed(t.n_var, t.n, 1000 + c);
-        gepi_panel *p = nullptr;
-        if ((rc = gepi_panel_create_bed(bed.data(), t.n_var, t.n, device, &p)) != GEPI_OK) break;
-        /* 4 phenotypes of about half cases each */
-        const int32_t w64 = (t.n + 63) / 64;
-        std::vector<std::vector<uint64_t>> m0(4, std::vector<uint64_t>((size_t)w64, 0)),
-            m1(4, std::vector<uint64_t>((size_t)w64, 0));
-        uint64_t s = 77 + (uint64_t)c;
-        for (int j = 0; j < 4; j++)
-            for (int32_t i = 0; i < t.n; i++) {
-                s = s * 6364136223846793005ull + 1442695040888963407ull;
-                ((s >> 40) & 1 ? m1 : m0)[(size_t)j][(size_t)(i / 64)] |= 1ull << (i % 64);
-            }
+    /* every class's panel and 4 phenotypes stay until the end: the table is
+     * checked on all 16 scans together */
+    std::vector<gepi_panel *> panels(4, nullptr);
+    std::vector<std::vector<uint64_t>> masks;
+    std::vector<ScanItem> all;
+    auto scans_of = [&](int c) {
         std::vector<ScanItem> items(4);
         for (int j = 0; j < 4; j++) {
-            items[(size_t)j].panel = p;
-            items[(size_t)j].mask_ctrl = m0[(size_t)j].data();
-            items[(size_t)j].mask_case = m1[(size_t)j].data();
+            items[(size_t)j].panel = panels[(size_t)c];
+            items[(size_t)j].mask_ctrl = masks[(size_t)(8 * c + 2 * j)].data();
+            items[(size_t)j].mask_case = masks[(size_t)(8 * c + 2 * j + 1)].data();
             items[(size_t)j].topmost = 100;
             items[(size_t)j].ffun = GEPI_FFUNC_ENTROPY;
         }
-        gepi_tune_class best{t.max_k4, 2, 2e9, 0.0, 0.0};
+        return items;
+    };
+    for (int c = 0; c < 4 && rc == GEPI_OK; c++) {
+        const Trial &t = trials[c];
+        const std::vector<uint8_t> bed = synth_bed(t.n_var, t.n, 1000 + c);
+        if ((rc = gepi_panel_create_bed(bed.data(), t.n_var, t.n, device, &panels[(size_t)c])) !=
+            GEPI_OK)
+            break;
+        /* 4 phenotypes of about half cases each */
+        const int32_t w64 = (t.n + 63) / 64;
+        uint64_t s = 77 + (uint64_t)c;
+        for (int j = 0; j < 4; j++) {
+            std::vector<uint64_t> m0((size_t)w64, 0), m1((size_t)w64, 0);
+            for (int32_t i = 0; i < t.n; i++) {
+                s = s * 6364136223846793005ull + 1442695040888963407ull;
+                ((s >> 40) & 1 ? m1 : m0)[(size_t)(i / 64)] |= 1ull << (i % 64);
+            }
+            masks.push_back(std::move(m0));
+            masks.push_back(std::move(m1));
+        }
+        std::vector<ScanItem> items = scans_of(c);
+        gepi_tune_class best{t.max_k4, 2, 0, 0, 2e9, 0.0, 0.0};
         double ms_best = INFINITY, ms_dflt = INFINITY, ms;
-        /* warm-up (allocations, tables, graphs) under the default */
-        tuning.assign(1, gepi_tune_class{INT32_MAX, 2, 2e9, 0.0, 0.0});
-        rc = time_batch(items, &ms);
+        /* warm-up (allocations, tables, graphs) under the default, then
+         * once per job shape (the first launches of its kernels) */
+        for (int tl : tiles) {
+            tuning.assign(1, gepi_tune_class{INT32_MAX, 2, tl, 0, 2e9, 0.0, 0.0});
+            if (rc == GEPI_OK) rc = time_batch(items, &ms);
+        }
         for (int r = 0; r < repeats && rc == GEPI_OK; r++)
-            for (int pc : per_cus)
-                for (double b : bands) {
-                    if (rc != GEPI_OK) break;
-                    tuning.assign(1, gepi_tune_class{INT32_MAX, pc, b, 0.0, 0.0});
-                    if ((rc = time_batch(items, &ms)) != GEPI_OK) break;
-                    if (pc == 2 && b == 2e9) ms_dflt = std::min(ms_dflt, ms);
-                    /* the best of each schedule over the repeats */
-                    if (ms < ms_best) {
-                        ms_best = ms;
-                        best.wg_per_cu = pc;
-                        best.band_evals = b;
+            for (int tl : tiles)
+                for (int pc : per_cus)
+                    for (double b : bands) {
+                        if (rc != GEPI_OK) break;
+                        tuning.assign(1, gepi_tune_class{INT32_MAX, pc, tl, 0, b, 0.0, 0.0});
+                        if ((rc = time_batch(items, &ms)) != GEPI_OK) break;
+                        if (tl == 0 && pc == 2 && b == 2e9) ms_dflt = std::min(ms_dflt, ms);
+                        /* the best of each schedule over the repeats */
+                        if (ms < ms_best) {
+                            ms_best = ms;
+                            best.wg_per_cu = pc;
+                            best.tile = tl;
+                            best.band_evals = b;
+                        }
                     }
-                }
-        gepi_panel_destroy(p);
         if (rc != GEPI_OK) break;
         if (ms_dflt <= 1.03 * ms_best) {
             /* within the trial's noise: keep the default */
             best.wg_per_cu = 2;
+            best.tile = 0;
             best.band_evals = 2e9;
             ms_best = ms_dflt;
         }
@@ -311,6 +343,39 @@ int Engine::autotune(int32_t repeats) {
         best.ms_default = ms_dflt;
         table.push_back(best);
     }
+    /* The classes interact in a mixed batch (a round's launch form follows
+     * its largest class; each job shape present is a launch of its own), so
+     * the table is kept only if it also beats the default table by 1% on
+     * all 16 scans together; otherwise every class gets the default.      */
+    if (rc == GEPI_OK) {
+        for (int c = 0; c < 4; c++) {
+            std::vector<ScanItem> items = scans_of(c);
+            all.insert(all.end(), items.begin(), items.end());
+        }
+        bool differs = false;
+        for (const gepi_tune_class &c : table)
+            differs |= c.wg_per_cu != 2 || c.tile != 0 || c.band_evals != 2e9;
+        double ms_t = INFINITY, ms_d = INFINITY, ms;
+        for (int r = 0; r < repeats && rc == GEPI_OK && differs; r++) {
+            tuning = table;
+            if ((rc = time_batch(all, &ms)) != GEPI_OK) break;
+            ms_t = std::min(ms_t, ms);
+            tuning.clear();
+            if ((rc = time_batch(all, &ms)) != GEPI_OK) break;
+            ms_d = std::min(ms_d, ms);
+        }
+        if (rc == GEPI_OK && differs && !(ms_t < 0.99 * ms_d))
+            for (gepi_tune_class &c : table) {
+                c.wg_per_cu = 2;
+                c.tile = 0;
+                c.band_evals = 2e9;
+                c.ms_best = c.ms_default;
+            }
+        mixed_ms_tuned_ = ms_t;
+        mixed_ms_default_ = ms_d;
+    }
+    for (gepi_panel *p : panels)
+        if (p) gepi_panel_destroy(p);
     tuning = rc == GEPI_OK ? table : std::vector<gepi_tune_class>();
     return rc;
 }
@@ -319,7 +384,7 @@ int Engine::grow_desc(size_t n) {
     if (n <= desc_cap_) return GEPI_OK;
     const size_t c = std::max<size_t>(n, 2 * desc_cap_);
     drop_graphs(); /* they hold the old tables' addresses */
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < NGROUP; i++) {
         hipFree(d_bdesc_[i]);
         hipFree(d_map_[i]);
         d_bdesc_[i] = nullptr;
@@ -332,7 +397,7 @@ int Engine::grow_desc(size_t n) {
     d_tdesc_ = nullptr;
     d_rep_ = nullptr;
     desc_cap_ = 0;
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < NGROUP; i++) {
         HIP_TRY(hipMalloc(&d_bdesc_[i], sizeof(BatchDesc) * c));
         HIP_TRY(hipMalloc(&d_map_[i], sizeof(int64_t) * 8 * (2 * c + 1)));
     }
@@ -474,6 +539,7 @@ int Engine::add(ScanItem *it) {
     }
     const gepi_tune_class *tc = tune_of(it->n_k4);
     const double band_evals = tc ? tc->band_evals : 2.0e9;
+    it->half = tc && tc->tile == 1;
     it->max_band = std::max<int64_t>(64, (int64_t)(band_evals / (double)F4_TILE_EVALS));
     it->band = std::max<int64_t>(1, std::min<int64_t>(it->max_band, (2 * it->k) / F4_TILE_EVALS + 1));
     it->forced_band = 0;
@@ -520,14 +586,15 @@ int Engine::round(std::vector<ScanItem *> &finished) {
         for (ScanItem *it : active_)
             it->band_used = std::max<int64_t>(1, (int64_t)((double)it->band_used * f));
     }
-    std::vector<BatchDesc> bd[2];
-    std::vector<ScanItem *> grp[2];
-    int64_t wg[2] = {0, 0};
+    /* count launch groups: objective (g & 1) x job shape (g >> 1) */
+    std::vector<BatchDesc> bd[NGROUP];
+    std::vector<ScanItem *> grp[NGROUP];
+    int64_t wg[NGROUP] = {0, 0, 0, 0};
     std::vector<ExactDesc> ed(na);
     std::vector<TopkDesc> td(na);
     for (size_t i = 0; i < na; i++) {
         ScanItem *it = active_[i];
-        grp[it->ffun == GEPI_FFUNC_ENTROPY ? 0 : 1].push_back(it);
+        grp[(it->ffun == GEPI_FFUNC_ENTROPY ? 0 : 1) + (it->half ? 2 : 0)].push_back(it);
         ExactDesc &e = ed[i];
         memset(&e, 0, sizeof e);
         e.e.st = it->d_state;
@@ -567,9 +634,11 @@ int Engine::round(std::vector<ScanItem *> &finished) {
     const int per_cu = f4_wg_per_cu(per_cu_dflt);
     /* captured rounds need a fixed grid: the persistent launch form */
     const bool graph = use_graph_ && per_cu > 0;
-    std::vector<int64_t> map[2];
-    for (int f = 0; f < 2; f++) {
+    std::vector<int64_t> map[NGROUP];
+    for (int f = 0; f < NGROUP; f++) {
         std::vector<ScanItem *> &g = grp[f];
+        /* half-tile jobs: two per tile, numbered 2 * tile + half */
+        const int64_t per_tile = f >= 2 ? 2 : 1;
         std::stable_sort(g.begin(), g.end(),
                          [](const ScanItem *a, const ScanItem *b) { return a->n_k4 > b->n_k4; });
         const size_t n = g.size();
@@ -583,8 +652,8 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             memset(&d, 0, sizeof d);
             d.g.planes_k = reinterpret_cast<const uint64_t *>(it->d_layout);
             d.g.pc3 = nullptr; /* three-K-step layout: no correction */
-            d.g.tile_begin = it->t0;
-            d.g.n_tiles = (int32_t)it->band_used;
+            d.g.tile_begin = it->t0 * per_tile;
+            d.g.n_tiles = (int32_t)(it->band_used * per_tile);
             d.g.n_var = (int32_t)it->n_var;
             d.g.n_var_pad = (int32_t)it->n_var_pad;
             d.g.n_ks = it->n_k4;
@@ -597,7 +666,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             d.g.pcap = it->pcap;
             d.g.lfact = tabs_.lfact;
             bd[f].push_back(d);
-            const int64_t b = it->band_used, r = dealt & 7;
+            const int64_t b = it->band_used * per_tile, r = dealt & 7;
             int64_t start = 0;
             for (int x = 0; x < 8; x++) {
                 const int64_t chunk = b / 8 + (((x - r) & 7) < (b & 7) ? 1 : 0);
@@ -619,7 +688,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
         ed.resize(desc_cap_);
         td.resize(desc_cap_);
     }
-    for (int f = 0; f < 2; f++) {
+    for (int f = 0; f < NGROUP; f++) {
         if (!bd[f].empty())
             HIP_TRY(hipMemcpyAsync(d_bdesc_[f], bd[f].data(), sizeof(BatchDesc) * bd[f].size(),
                                    hipMemcpyHostToDevice, stream));
@@ -631,16 +700,17 @@ int Engine::round(std::vector<ScanItem *> &finished) {
                            stream));
     HIP_TRY(hipMemcpyAsync(d_tdesc_, td.data(), sizeof(TopkDesc) * td.size(), hipMemcpyHostToDevice,
                            stream));
-    wq_parity_[0] = wq_parity_[1] = 0;
+    for (int f = 0; f < NGROUP; f++) wq_parity_[f] = 0;
     if (graph) {
         if (gx_cap_ != desc_cap_ || gx_per_cu_ != per_cu) {
             drop_graphs();
             gx_cap_ = desc_cap_;
             gx_per_cu_ = per_cu;
         }
-        const int mask = (wg[0] > 0 ? 1 : 0) | (wg[1] > 0 ? 2 : 0);
+        int mask = 0;
+        for (int f = 0; f < NGROUP; f++) mask |= wg[f] > 0 ? 1 << f : 0;
         if (!gx_count_[mask] && (rc = capture_graph(mask, per_cu)) != GEPI_OK) return rc;
-        if (!gx_finish_ && (rc = capture_graph(4, per_cu)) != GEPI_OK) return rc;
+        if (!gx_finish_ && (rc = capture_graph(NGRAPH_COUNT, per_cu)) != GEPI_OK) return rc;
         /* the count graph clears the work counters first */
         HIP_TRY(hipEventRecord(ev0_, stream));
         HIP_TRY(hipGraphLaunch(gx_count_[mask], stream));
@@ -650,9 +720,9 @@ int Engine::round(std::vector<ScanItem *> &finished) {
     } else {
         /* work counters start clean every round (a failed round leaves no
          * stale counts behind) */
-        HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 32, stream));
+        HIP_TRY(hipMemsetAsync(d_wq_, 0, sizeof(uint32_t) * 16 * NGROUP, stream));
         HIP_TRY(hipEventRecord(ev0_, stream));
-        for (int f = 0; f < 2; f++) {
+        for (int f = 0; f < NGROUP; f++) {
             if (wg[f] == 0) continue;
             const size_t n = bd[f].size();
             BatchMap m;
@@ -671,17 +741,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
                     std::max<int64_t>(8, (per_cu * (int64_t)device_cus(device)) / 8 * 8),
                     ((wg[f] + 7) / 8) * 8);
             }
-            if (f == 0) {
-                if (per_cu > 0)
-                    k_count_f4_batch<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-                else
-                    k_count_f4_batch<GEPI_FFUNC_ENTROPY, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-            } else {
-                if (per_cu > 0)
-                    k_count_f4_batch<GEPI_FFUNC_FISHERPV, true><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-                else
-                    k_count_f4_batch<GEPI_FFUNC_FISHERPV, false><<<grid, F4_THREADS, 0, stream>>>(d_bdesc_[f], m);
-            }
+            launch_count(f, per_cu > 0, grid, m);
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(ev1_, stream));
@@ -800,6 +860,8 @@ extern "C" int gepi_engine_set_tuning(gepi_engine *e, const gepi_tune_class *cla
             return gepi_fail(GEPI_ERR_ARG, "class %d: max_ksteps must increase", i);
         if (c.wg_per_cu < 0 || c.wg_per_cu > 16)
             return gepi_fail(GEPI_ERR_ARG, "class %d: wg_per_cu %d outside [0, 16]", i, c.wg_per_cu);
+        if (c.tile != 0 && c.tile != 1)
+            return gepi_fail(GEPI_ERR_ARG, "class %d: tile %d is neither 0 nor 1", i, c.tile);
         if (!(c.band_evals >= 1e6) || !std::isfinite(c.band_evals))
             return gepi_fail(GEPI_ERR_ARG, "class %d: band_evals below 1e6", i);
     }

@@ -621,9 +621,16 @@ __device__ __forceinline__ F4Job uniform_job(const F4Job &j) {
     return o;
 }
 
-template <int FFUN, class Src>
+template <int FFUN, bool HALF, class Src>
 __device__ __forceinline__ void count_f4_run(const Src &src) {
-    __shared__ __attribute__((aligned(16))) uint4 lds[2][F4_UNITS];
+    /* HALF: a job is half of a 64 x 32-variant tile, its A variants 32h ..
+     * 32h + 31 (job unit 2 * tile + h), and each wave takes 16 x 16
+     * variants (one A block); else a job is the whole tile and each wave
+     * 32 x 16 (two A blocks).  Same tile numbering and bands either way. */
+    constexpr int IB = HALF ? 1 : 2;        /* A blocks of 16 variants per wave */
+    constexpr int RA = HALF ? F4_RA / 2 : F4_RA; /* A plane rows of a job        */
+    constexpr int UNITS = (RA + TB4 * 3) * 4;    /* uint4 of a staged super-step  */
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][UNITS];
     __shared__ PendStage stage;
     __shared__ F4Job s_job[2];
 
@@ -632,36 +639,41 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
     const int wa = wave >> 1, wb = wave & 1;
     const int l16 = lane & 15, lg = lane >> 4;
 
-    /* Staging.  A super-step of a tile is 288 plane rows of 64 bytes (4
-     * uint4): A rows 0..191 (variant 64ta + r/3, plane r%3) and B rows, both
-     * contiguous in the layout; thread t stores LDS units t + 256j of the A
-     * rows (j < 3) and of the B rows (t + 256j < 384), see f4_unit.        */
+    /* Staging.  A super-step of a job is RA + 96 plane rows of 64 bytes (4
+     * uint4): A rows 0..RA-1 (plane r%3 of the job's A variant r/3) and B
+     * rows, both contiguous in the layout; thread t stores LDS units t +
+     * 256j of the A rows (t + 256j < 4 RA) and of the B rows (t + 256j <
+     * 384), see f4_unit (RA and 192 are multiples of 16 rows, so the
+     * swizzle of f4_src holds for every block).                           */
     const int si = f4_src(tid);
     const bool hasB1 = tid < 128;
-    const int dA = tid, dB = F4_RA * 4 + tid;
+    const bool hasA1 = !HALF || tid < 128;
+    const int dA = tid, dB = RA * 4 + tid;
     constexpr int D64 = 256; /* 64 rows further down */
-    /* lane's fragment rows: A (i, p) is row (32wa + 16i + l16)*3 + p, B q is
-     * row 192 + (16wb + l16)*3 + q; the lane takes 16-byte column lg.      */
+    /* lane's fragment rows: A (i, p) is row (16 IB wa + 16i + l16)*3 + p, B q
+     * is row RA + (16wb + l16)*3 + q; the lane takes 16-byte column lg.    */
     int oA[3], oB[3]; /* A (i, p) at oA[p] + 192 i: 48 rows keep the swizzle */
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-        oA[p] = f4_unit((32 * wa + l16) * 3 + p, lg);
-        oB[p] = f4_unit(F4_RA + (16 * wb + l16) * 3 + p, lg);
+        oA[p] = f4_unit((16 * IB * wa + l16) * 3 + p, lg);
+        oB[p] = f4_unit(RA + (16 * wb + l16) * 3 + p, lg);
     }
 
     /* geometry of one tile */
     struct Geo {
         const char *tA, *tB; /* the tile's A and B rows of super-step 0 (uniform) */
         int64_t ustride_b;   /* bytes per super-step */
-        int32_t ta, tb, u1, n_u, last, ks1;
+        int32_t ta, tb, a0, u1, n_u, last, ks1; /* a0: the job's first A variant in the tile */
     };
     const uint32_t so = (uint32_t)si * 16u;
-    auto geo_of = [&](const CountArgs &g, int64_t tile) {
+    auto geo_of = [&](const CountArgs &g, int64_t unit) {
         Geo o;
+        const int64_t tile = HALF ? unit >> 1 : unit;
         tile_of_f4(tile, g.n_var_pad / TM, o.ta, o.tb);
+        o.a0 = HALF ? (int32_t)(unit & 1) * 32 : 0;
         const char *lay = reinterpret_cast<const char *>(g.planes_k);
         o.ustride_b = (int64_t)g.n_var_pad * 3 * 64;
-        o.tA = lay + (int64_t)o.ta * (F4_RA * 64);
+        o.tA = lay + ((int64_t)o.ta * TM + o.a0) * (3 * 64);
         o.tB = lay + (int64_t)o.tb * (TB4 * 3 * 64);
         if constexpr (FFUN != GEPI_FFUNC_ENTROPY) {
             /* Fisher: per-thread pointers (see fetch) */
@@ -684,8 +696,8 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             const uint4 *a = reinterpret_cast<const uint4 *>(o.tA + off);
             const uint4 *b = reinterpret_cast<const uint4 *>(o.tB + off);
             pa0 = a[0];
-            pa1 = a[256];
-            pa2 = a[512];
+            if (hasA1) pa1 = a[256];
+            if (!HALF) pa2 = a[512];
             pb0 = b[0];
             if (hasB1) pb1 = b[256];
             return;
@@ -693,19 +705,19 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         /* scalar buffer resources over the super-step's A and B rows, as in
          * count_f4_tile */
         const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<char *>(o.tA + off), (short)0, F4_RA * 64, F4_RSRC_WORD3);
+            const_cast<char *>(o.tA + off), (short)0, RA * 64, F4_RSRC_WORD3);
         const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char *>(o.tB + off), (short)0, TB4 * 3 * 64, F4_RSRC_WORD3);
         pa0 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so, 0, 0));
-        pa1 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so + 4096, 0, 0));
-        pa2 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so + 8192, 0, 0));
+        if (hasA1) pa1 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so + 4096, 0, 0));
+        if (!HALF) pa2 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, (int)so + 8192, 0, 0));
         pb0 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(rb, (int)so, 0, 0));
         if (hasB1) pb1 = as_u4(__builtin_amdgcn_raw_buffer_load_b128(rb, (int)so + 4096, 0, 0));
     };
     auto put = [&](uint4 *W) {
         W[dA] = pa0;
-        W[dA + D64] = pa1;
-        W[dA + 2 * D64] = pa2;
+        if (hasA1) W[dA + D64] = pa1;
+        if (!HALF) W[dA + 2 * D64] = pa2;
         W[dB] = pb0;
         if (hasB1) W[dB + D64] = pb1;
     };
@@ -731,14 +743,14 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         /* super-step u into one class's accumulators, its first np K steps
          * (the rest of a class's last super-step is padding); the staging
          * pipeline runs on across the two phases.                          */
-        auto step = [&](v4f (&acc)[2][3][3], int32_t u, int32_t np) {
+        auto step = [&](v4f (&acc)[IB][3][3], int32_t u, int32_t np) {
             const uint4 *L = lds[u & 1];
-            uint4 wA[2][3], wB[3];
+            uint4 wA[IB][3], wB[3];
             /* B first: the first MFMA needs B and A[0][0] only */
 #pragma unroll
             for (int q = 0; q < 3; q++) wB[q] = L[oB[q]];
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < IB; i++)
 #pragma unroll
                 for (int p = 0; p < 3; p++) wA[i][p] = L[oA[p] + 192 * i];
 #pragma unroll
@@ -749,7 +761,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
 #pragma unroll
                     for (int q = 0; q < 3; q++) b[q] = frag_f4(wB[q], m);
 #pragma unroll
-                    for (int i = 0; i < 2; i++)
+                    for (int i = 0; i < IB; i++)
 #pragma unroll
                         for (int p = 0; p < 3; p++) {
                             const v8i a = frag_f4(wA[i][p], m);
@@ -768,9 +780,9 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             fetch(geo, u + 2);
             __syncthreads();
         };
-        v4f acc0[2][3][3], acc1[2][3][3];
+        v4f acc0[IB][3][3], acc1[IB][3][3];
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < IB; i++)
 #pragma unroll
             for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -780,7 +792,7 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
             step(acc0, u, r < 3 ? r : 3);
         }
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < IB; i++)
 #pragma unroll
             for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -805,12 +817,12 @@ __device__ __forceinline__ void count_f4_run(const Src &src) {
         }
 
         /* filter: lane holds D[4*lg + r][l16] of each 16x16 block, i.e. A
-         * variant 32wa + 16i + 4lg + r against B variant 16wb + l16.       */
+         * variant a0 + 16 IB wa + 16i + 4lg + r against B variant 16wb + l16. */
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
+        for (int i = 0; i < IB; i++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int32_t A = geo.ta * TM + 32 * wa + 16 * i + 4 * lg + r;
+                const int32_t A = geo.ta * TM + geo.a0 + 16 * IB * wa + 16 * i + 4 * lg + r;
                 const int32_t B = geo.tb * TB4 + 16 * wb + l16;
                 if (A >= g.n_var || B > A) continue;
                 if (g.f32) {

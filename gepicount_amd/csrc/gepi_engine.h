@@ -1,7 +1,8 @@
 /*
  * gepi_engine.h -- the batched scan engine (host side).  Scans are added at
  * any time and advance together in rounds; a round is one count launch per
- * objective, one exact-score launch and one top-k launch for all active
+ * objective and job shape present, one exact-score launch and one top-k
+ * launch for all active
  * scans, then a single read of their reports; with GEPI_ENGINE_GRAPH=1 a
  * round's launches are replayed from two captured HIP graphs.  The batch C
  * ABI runs rounds
@@ -22,6 +23,7 @@
 
 namespace gepi {
 struct BatchDesc;
+struct BatchMap;
 struct ExactDesc;
 struct TopkDesc;
 struct ItemReport;
@@ -47,6 +49,7 @@ struct ScanItem {
     int64_t k = 0, n_var = 0, n_var_pad = 0, n_tiles = 0, t0 = 0;
     int64_t band = 1, band_used = 0, max_band = 1, forced_band = 0;
     int32_t n0 = 0, n1 = 0, n_k4 = 0, ks0 = 0;
+    bool half = false; /* count jobs are half tiles (its class's tile shape 1) */
     unsigned long long cap = 0, pcap = 0, count = 0, exact_total = 0;
     uint4 *d_layout = nullptr;
     gepi::CompressWord *d_cw = nullptr;
@@ -85,6 +88,9 @@ class Engine {
     std::vector<gepi_tune_class> tuning;
     /* Measures the table on synthetic batches (gepi_engine_autotune). */
     int autotune(int32_t repeats);
+    /* The last autotune's check on all trial scans together (ms; infinite
+     * when the per-class table equalled the default).                     */
+    double mixed_ms_tuned_ = 0.0, mixed_ms_default_ = 0.0;
 
   private:
     void *dalloc(size_t bytes);
@@ -98,11 +104,14 @@ class Engine {
      * until idle.                                                           */
     int time_batch(std::vector<ScanItem> &items, double *ms);
     /* Captures part of a round for descriptor capacity desc_cap_ and a
-     * persistent grid of per_cu workgroups per CU: part 0-3 into
+     * persistent grid of per_cu workgroups per CU: part 0-15 into
      * gx_count_[part] (work-counter clear, then the count launches of the
-     * objectives in bit mask part: 1 entropy, 2 Fisher), part 4 into
-     * gx_finish_ (exact + top-k launches).                                  */
+     * groups in bit mask part: bit g = objective (g & 1: 0 entropy, 1
+     * Fisher) and job shape (g >> 1: 0 whole tiles, 1 half tiles)), part
+     * NGRAPH_COUNT into gx_finish_ (exact + top-k launches).                */
     int capture_graph(int part, int per_cu);
+    /* One count launch of group g (as in capture_graph). */
+    void launch_count(int g, bool persist, unsigned grid, const gepi::BatchMap &m);
     void drop_graphs();
 
     DeviceTables tabs_;
@@ -110,20 +119,22 @@ class Engine {
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
     size_t in_use_ = 0;
-    gepi::BatchDesc *d_bdesc_[2] = {nullptr, nullptr};
-    int64_t *d_map_[2] = {nullptr, nullptr};
+    /* count launch groups: objective x job shape */
+    static constexpr int NGROUP = 4, NGRAPH_COUNT = 16;
+    gepi::BatchDesc *d_bdesc_[NGROUP] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t *d_map_[NGROUP] = {nullptr, nullptr, nullptr, nullptr};
     gepi::ExactDesc *d_edesc_ = nullptr;
     gepi::TopkDesc *d_tdesc_ = nullptr;
     gepi::ItemReport *d_rep_ = nullptr;
     size_t desc_cap_ = 0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-    uint32_t *d_wq_ = nullptr; /* fp4 work counters: [objective][2 sets][8] */
-    int32_t wq_parity_[2] = {0, 0};
+    uint32_t *d_wq_ = nullptr; /* fp4 work counters: [group][2 sets][8] */
+    int32_t wq_parity_[NGROUP] = {0, 0, 0, 0};
     /* rounds as captured graphs (GEPI_ENGINE_GRAPH=1; else launched one by
      * one); valid for gx_cap_ descriptor slots and gx_per_cu_ workgroups
      * per CU                                                                 */
     bool use_graph_ = false;
-    hipGraphExec_t gx_count_[4] = {nullptr, nullptr, nullptr, nullptr}, gx_finish_ = nullptr;
+    hipGraphExec_t gx_count_[NGRAPH_COUNT] = {}, gx_finish_ = nullptr;
     size_t gx_cap_ = 0;
     int gx_per_cu_ = 0;
 };

@@ -242,16 +242,18 @@ def test_batch_autotune_and_tables(monkeypatch):
         print("autotune table", table, flush=True)
         assert [c["max_ksteps"] for c in table] == [2, 8, 32, 2**31 - 1]
         for c in table:
-            assert c["wg_per_cu"] in (0, 1, 2) and c["band_evals"] in (5e8, 2e9, 8e9), c
+            assert c["wg_per_cu"] in (0, 1, 2, 3) and c["band_evals"] in (5e8, 2e9, 8e9), c
+            assert c["tile"] in (0, 1), c
             assert 0 < c["ms_best"] <= c["ms_default"], c
         tuned, _ = run_batch(panels, items, eng)
-        eng.set_tuning([dict(max_ksteps=8, wg_per_cu=1, band_evals=1e6),
+        eng.set_tuning([dict(max_ksteps=8, wg_per_cu=1, tile=1, band_evals=1e6),
                         dict(max_ksteps=20, wg_per_cu=0, band_evals=3e6),
-                        dict(max_ksteps=2**31 - 1, wg_per_cu=2, band_evals=3e6)])
+                        dict(max_ksteps=2**31 - 1, wg_per_cu=2, tile=1, band_evals=3e6)])
         extreme, bs2 = run_batch(panels, items, eng)
         assert bs2["rounds"] >= bs0["rounds"]
         for bad in ([dict(max_ksteps=8, wg_per_cu=2, band_evals=2e9)] * 2,
                     [dict(max_ksteps=8, wg_per_cu=-1, band_evals=2e9)],
+                    [dict(max_ksteps=8, wg_per_cu=2, tile=2, band_evals=2e9)],
                     [dict(max_ksteps=8, wg_per_cu=2, band_evals=10.0)]):
             with pytest.raises(ge.GepiError):
                 eng.set_tuning(bad)
@@ -264,3 +266,44 @@ def test_batch_autotune_and_tables(monkeypatch):
         for label, res in (("default", base), ("tuned", tuned), ("extreme", extreme), ("reset", again)):
             assert res[i][2] == 0, (label, i, res[i][2])
             assert_same(res[i][0], ref, "%s item %d vs oracle" % (label, i))
+
+
+@pytest.mark.parametrize("per_cu", [0, 2, 3])
+def test_batch_half_tile_jobs(per_cu, monkeypatch):
+    """Scans whose class runs half-tile jobs (32 x 32 variants, 16 x 16 per
+    wave) next to scans on whole tiles, both objectives, so one round has up
+    to four count launches; every item equals the oracle.  The last two
+    variants also force overflows and replay captured-graph rounds."""
+    print("case batch half tiles per_cu=%d" % per_cu, flush=True)
+    monkeypatch.setenv("GEPI_F4_WG_PER_CU", str(per_cu))
+    panels, items = make_mix(61 + per_cu, 12, vmax=700, nmax=1400)
+    table = [dict(max_ksteps=6, wg_per_cu=per_cu, tile=1, band_evals=2e9),
+             dict(max_ksteps=2**31 - 1, wg_per_cu=per_cu, tile=0, band_evals=2e9)]
+    with ge.Engine() as eng:
+        eng.set_tuning(table)
+        res, _ = run_batch(panels, items, eng)
+        if per_cu == 3:
+            big = [dict(it, topmost=20000 if it["ffun"] == 0 else 2000) for it in items]
+            monkeypatch.setenv("GEPI_CAND_CAP", "1")
+            monkeypatch.setenv("GEPI_PEND_CAP", "4000000")
+            monkeypatch.setenv("GEPI_BAND_TILES", "8")
+            ovf, _ = run_batch(panels, big, eng)
+            for v in ("GEPI_CAND_CAP", "GEPI_PEND_CAP", "GEPI_BAND_TILES"):
+                monkeypatch.delenv(v)
+            assert sum(st["cand_overflows"] for _, st, _ in ovf) > 0
+    runs = [("half", res, items)] + ([("overflow", ovf, big)] if per_cu == 3 else [])
+    if per_cu == 2:
+        monkeypatch.setenv("GEPI_ENGINE_GRAPH", "1")
+        with ge.Engine() as eng:
+            eng.set_tuning(table)
+            g1, bs = run_batch(panels, items, eng)
+            g2, _ = run_batch(panels, items[::-1], eng)
+        assert bs["graph_rounds"] > 0
+        runs += [("graph", g1, items), ("graph reversed", g2[::-1], items)]
+    for label, r, its in runs:
+        for i, it in enumerate(its):
+            bits, valid, n = panels[it["panel_index"]]
+            ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], n),
+                                it["topmost"], ffun=it["ffun"])
+            assert r[i][2] == 0, (label, i, r[i][2])
+            assert_same(r[i][0], ref, "%s item %d vs oracle" % (label, i))
