@@ -32,7 +32,7 @@ EXPORTS = ("gepi_params_init", "gepi_process_gpu_1", "gepi_panel_create", "gepi_
            "gepi_panel_filter", "gepi_plink_open",
            "gepi_plink_close", "gepi_plink_check_quality", "gepi_print_interactions",
            "gepi_fisher_pval", "gepi_score_table", "gepi_last_error", "gepi_version",
-           "gepi_device_count",
+           "gepi_device_count", "gepi_set_progress",
            "gepi_engine_create", "gepi_engine_destroy", "gepi_engine_scan_batch",
            "gepi_engine_autotune", "gepi_engine_set_tuning",
            "gepi_server_create", "gepi_server_destroy", "gepi_server_add_panel_bed",
@@ -173,6 +173,7 @@ def lib():
     l.gepi_last_error.restype = ctypes.c_char_p
     l.gepi_version.restype = ctypes.c_char_p
     l.gepi_device_count.argtypes = [ctypes.POINTER(ctypes.c_int32)]
+    l.gepi_set_progress.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     l.gepi_score_table.argtypes = [ctypes.c_int32] * 5 + [ctypes.POINTER(ctypes.c_double)]
     l.gepi_engine_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp)]
     l.gepi_engine_destroy.argtypes = [vp]

@@ -30,7 +30,8 @@ void print_help() {
             "   -f | --fisher-pvals         Computes Fisher P-value instead of Entropy [no]\n"
             "   -G | --gpu-num         N    Use GPU number N  [0]\n"
             "   -t | --alternate-grid       Accepted for compatibility (no effect)\n"
-            "   -D | --print-diagonal       Print a sample of the ranking (stride 100000) [no]\n"
+            "   -D | --print-diagonal       Print progress and a sample of the current ranking\n"
+            "                               (stride 100000) after every band [no]\n"
             "   -N | --topmost         M    Maintain a list of M results  [%d]\n\n"
             "        --block-sizes     N M  Accepted for compatibility; N*M must be a multiple of 32 [16,16]\n"
             "        --max-grid-area   N    Cap evaluations per kernel launch at N x 1024 x 1024 [auto]\n"
@@ -45,6 +46,37 @@ void print_help() {
 }
 
 bool need(int j, int argc, int extra) { return j + extra < argc; }
+
+/* -D: progress and a strided sample of the current ranking after every band,
+ * as the reference's tile loop prints them (src/kernels/kernel1.cu:420-438;
+ * stride and context of src/misc/print_misc.h:30-31).                      */
+struct ProgressCtx {
+    const gepi_plink *pr;
+    const int64_t *kept; /* --qa-remove: original variant of each scanned one */
+};
+
+void print_progress(const gepi_progress *pg, void *user) {
+    const ProgressCtx *ctx = static_cast<const ProgressCtx *>(user);
+    const double total_s = pg->ms_elapsed / 1000.0;
+    const double pct = pg->tiles_total > 0 ? 100.0 * (double)pg->tiles_done / (double)pg->tiles_total
+                                           : 100.0;
+    double rem = pg->ms_remaining / 1000.0;
+    const char *unit = "sec";
+    if (rem > 3600.0) {
+        rem /= 3600.0;
+        unit = "hr";
+    }
+    fprintf(stderr, "K1: Total time %gsec (%g%%) Remaining: %g%s\n", total_s, pct, rem, unit);
+    fprintf(stderr, "K1: Last processed: tile %lld of %lld\n", (long long)pg->tiles_done,
+            (long long)pg->tiles_total);
+    std::vector<gepi_result> tops(pg->tops, pg->tops + pg->n_tops);
+    if (ctx->kept)
+        for (gepi_result &r : tops) {
+            r.sA = 2 * ctx->kept[r.sA / 2] + (r.sA & 1);
+            r.sB = 2 * ctx->kept[r.sB / 2] + (r.sB & 1);
+        }
+    gepi_print_interactions(ctx->pr, tops.data(), (int64_t)tops.size(), 100000, 5);
+}
 
 } // namespace
 
@@ -138,6 +170,8 @@ int main(int argc, char **argv) {
     std::vector<gepi_result> tops((size_t)(par.topmost > 0 ? par.topmost : 1));
     int64_t n = 0;
     gepi_scan_stats st;
+    ProgressCtx progress{pr, nullptr};
+    if (diag) gepi_set_progress(print_progress, &progress);
     gepi_panel *panel = nullptr;
     int rc = gepi_panel_create_bed(pr->bed_payload, pr->n_rows / 2, pr->n_samples, par.gpu_num,
                                    &panel);
@@ -159,6 +193,7 @@ int main(int argc, char **argv) {
             }
             printf("QA removal: %lld of %lld SNPs kept\n", (long long)nk,
                    (long long)(pr->n_rows / 2));
+            progress.kept = idx.data();
             rc = gepi_scan(kept, m0.data(), m1.data(), &par, tops.data(), &n, &st);
             gepi_panel_destroy(kept);
             for (int64_t i = 0; i < n && rc == GEPI_OK; i++) {
@@ -183,7 +218,6 @@ int main(int argc, char **argv) {
             par.ffun == GEPI_FFUNC_ENTROPY ? "ENTROPY" : "FISHER PV", (long long)st.evals,
             st.n_class0, st.n_class1, st.count_path, st.bands, st.ms_scan, st.ms_count_kernels,
             st.ms_prepare, st.ms_select, (long long)st.exact_evals);
-    if (diag) gepi_print_interactions(pr, tops.data(), n, 100000, 5);
     gepi_print_interactions(pr, tops.data(), n, 1, 0);
     gepi_plink_close(pr);
     printf("Cleaned\n");

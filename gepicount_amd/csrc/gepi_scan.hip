@@ -346,6 +346,10 @@ TopkDesc topk_desc(const ScanBufs &b, int cur, unsigned long long cap, int64_t k
     return d;
 }
 
+/* the calling thread's progress hook (gepi_set_progress) */
+thread_local gepi_progress_fn g_progress_fn = nullptr;
+thread_local void *g_progress_user = nullptr;
+
 double ms_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
@@ -721,6 +725,25 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         band = std::max<int64_t>(1, std::min<int64_t>({max_band, next, band * 4}));
         if (forced_band) band = std::min<int64_t>(forced_band, (t1 - t0) * 4);
         t0 = t1;
+        if (par->intermediate && g_progress_fn) {
+            /* the records held so far, sorted, for the caller's report */
+            std::vector<Cand> held((size_t)count);
+            if (count > 0)
+                HIP_TRY(hipMemcpyAsync(held.data(), b.cand[cur], sizeof(Cand) * count,
+                                       hipMemcpyDeviceToHost, p->stream));
+            HIP_TRY(hipStreamSynchronize(p->stream));
+            std::vector<gepi_result> cur_tops((size_t)std::min<int64_t>(k, (int64_t)count));
+            gepi_progress pg;
+            memset(&pg, 0, sizeof pg);
+            pg.bands = bands;
+            pg.tiles_done = t0;
+            pg.tiles_total = n_tiles;
+            pg.ms_elapsed = ms_since(t_start);
+            pg.ms_remaining = pg.ms_elapsed * (double)(n_tiles - t0) / (double)t0;
+            pg.n_tops = candidates_to_results(held, k, cur_tops.data());
+            pg.tops = cur_tops.data();
+            g_progress_fn(&pg, g_progress_user);
+        }
     }
     /* final extraction: exactly k records, sorted by key on the host */
     auto ts = std::chrono::steady_clock::now();
@@ -783,6 +806,12 @@ extern "C" void gepi_params_init(gepi_params *par) {
     par->count_path = 0;
     par->blk_x = par->blk_y = 16;
     par->max_grid_area = 0;
+}
+
+extern "C" int gepi_set_progress(gepi_progress_fn fn, void *user) {
+    g_progress_fn = fn;
+    g_progress_user = fn ? user : nullptr;
+    return GEPI_OK;
 }
 
 extern "C" int gepi_device_count(int32_t *count) {

@@ -72,7 +72,8 @@ typedef struct gepi_params {
     int64_t topmost;       /* -N: results to keep                              */
     int32_t count_path;    /* 0 = fastest (fp4 MFMA), 1 = popcount (VALU),     
                             * 2 = int8 MFMA, 3 = fp4 (e2m1) MFMA                */
-    int32_t intermediate;  /* -D: caller prints intermediate results (CLI)     */
+    int32_t intermediate;  /* -D: report progress and intermediate results
+                            * through the thread's progress hook              */
     int32_t blk_x, blk_y;  /* --block-sizes: accepted for compatibility        */
     int32_t max_grid_area; /* --max-grid-area: cap on evals per launch, in Mi  */
     int32_t reserved;
@@ -97,6 +98,27 @@ typedef struct gepi_scan_stats {
 
 /* Defaults of src/gEpiCount.cpp:34-39 and :100-115. */
 void gepi_params_init(gepi_params *par);
+
+/* Progress of a running scan, for the reference's -D output: its tile loop
+ * reports elapsed and remaining time and prints the current best records
+ * every so often (src/kernels/kernel1.cu:420-438,
+ * src/proc_gold_1.h:140-163).  Here a scan with par->intermediate set calls
+ * the calling thread's progress hook after every band of tiles.           */
+typedef struct gepi_progress {
+    int32_t bands;           /* bands completed                               */
+    int32_t reserved;
+    int64_t tiles_done, tiles_total;
+    double ms_elapsed;       /* since the scan started                        */
+    double ms_remaining;     /* estimate: the same time per tile for the rest */
+    const gepi_result *tops; /* the best records so far, sorted ascending     */
+    int64_t n_tops;          /* min(topmost, records held)                    */
+} gepi_progress;
+typedef void (*gepi_progress_fn)(const gepi_progress *progress, void *user);
+/* Installs (fn != NULL) or removes the calling thread's progress hook; it
+ * applies to gepi_scan and gepi_process_gpu_1 calls made by this thread
+ * with par->intermediate != 0.  Copying the records to the host costs a
+ * few ms per band on large top-k, so it is a diagnostic, not for timing.  */
+int gepi_set_progress(gepi_progress_fn fn, void *user);
 
 /* Plane-level scan on device `par->gpu_num`.  Mirrors
  *   void process_gpu_1(PlinkReader<ui8> *pr, InteractionResult *tops,

@@ -382,6 +382,32 @@ def test_cli_report_equals_hand_derived_kat_report():
     assert r.stderr[start:start + len(KAT_REPORT_N10)] == KAT_REPORT_N10
 
 
+def test_cli_intermediate_progress():
+    """-D: after every band the CLI prints the reference's progress lines and
+    a strided sample of the current ranking; the final report is unchanged."""
+    print("case cli -D", flush=True)
+    root = os.path.join(DATA, "thinhap1")
+    exe = os.path.join(ROOT, "gepicount_amd", "gepicount")
+    env = dict(os.environ, GEPI_BAND_TILES="4")
+    r = subprocess.run([exe, "-i", root, "-N", "100", "-D"], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = r.stderr.splitlines()
+    progress = [l for l in lines if l.startswith("K1: Total time ")]
+    assert len(progress) >= 3, len(progress)
+    assert progress[-1].startswith("K1: Total time ") and "(100%)" in progress[-1]
+    header = ("Rank:\tidxA[id allele chr:pos]\tFunc\tidxB[id allele chr:pos]\tObj\tFPv\t"
+              "(c00,c01,c10,c11)")
+    starts = [i for i, l in enumerate(lines) if l == header]
+    assert len(starts) == len(progress) + 1  # one sample per band, then the report
+    report = lines[starts[-1] - 1:starts[-1] + 101]
+    gold = subprocess.run([os.path.join(ROOT, "oracle", "gepicount_gold"), "-i", root, "-N", "100"],
+                          capture_output=True, text=True, timeout=120)
+    assert report == gold.stderr.splitlines()[:102]
+    # the last intermediate sample shows the final ranking's first records
+    assert lines[starts[-2] + 1] == report[2]
+
+
 def test_cli_report_equals_oracle_report():
     root = os.path.join(DATA, "thinhap1")
     exe = os.path.join(ROOT, "gepicount_amd", "gepicount")
