@@ -447,9 +447,10 @@ def main():
     path = stats[-1]["count_path"]
     k_ms = statistics.mean(s["ms_count_kernels"] for s in stats)
     launches = statistics.mean(s["bands"] for s in stats)
-    if path == 3:
+    if path in (3, 4):
         ops_total = var_pairs * 9 * 2.0 * (n0 + n1)
-        peak, bound, unit, kernel = FP4_MFMA_PEAK_FLOPS, "mfma", "TFLOP/s", "k_count_f4"
+        peak, bound, unit, kernel = (FP4_MFMA_PEAK_FLOPS, "mfma", "TFLOP/s",
+                                     "k_count_f4w" if path == 3 else "k_count_f4")
     elif path == 2:
         ops_total = var_pairs * 9 * 2.0 * (n0 + n1)
         peak, bound, unit, kernel = INT8_MFMA_PEAK_OPS, "mfma", "TOP/s", "k_count_mfma"

@@ -36,7 +36,7 @@ void print_help() {
             "        --block-sizes     N M  Accepted for compatibility; N*M must be a multiple of 32 [16,16]\n"
             "        --max-grid-area   N    Cap evaluations per kernel launch at N x 1024 x 1024 [auto]\n"
             "        --shared-memory        Accepted for compatibility (tiles always use LDS)\n"
-            "        --count-path      P    popcount | mfma | fp4 | auto  [auto]\n"
+            "        --count-path      P    popcount | mfma | fp4 | fp4rows | auto  [auto]\n"
             "        --miss-snps       F    Max missing values fraction for SNPs [0.05]\n"
             "        --miss-samples    F    Max missing values fraction for samples  [0.05]\n"
             "        --qa-remove            Drop SNPs and samples beyond those fractions before\n"
@@ -114,6 +114,7 @@ int main(int argc, char **argv) {
             if (v == "popcount") par.count_path = 1;
             else if (v == "mfma") par.count_path = 2;
             else if (v == "fp4") par.count_path = 3;
+            else if (v == "fp4rows") par.count_path = 4;
             else if (v == "auto") par.count_path = 0;
             else {
                 fprintf(stderr, "Unknown count path: %s\n", v.c_str());

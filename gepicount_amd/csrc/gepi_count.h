@@ -882,6 +882,173 @@ __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
     count_f4_tile<FFUN, F32>(g, band_tile(g.tile_begin, g.n_tiles));
 }
 
+/* ---------------------------------------------------------------------- */
+/* fp4 wave-tile path (single scans)                                      */
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int F4W_THREADS = 256; /* 4 waves: two 64 x 32-variant tiles, 32 x 32 each */
+
+/* Class-0 counts of a wave tile, two 16-bit counts per dword (in LDS). */
+constexpr int F4W_PARK = 72;
+static_assert(GEPI_MAX_SAMPLES <= 65535, "a class count must fit 16 bits");
+
+/* Workgroup -> its two tiles (2u, 2u + 1 of the band), units numbered as
+ * band_tile numbers tiles: each XCD takes a contiguous run of units.      */
+__device__ __forceinline__ int64_t band_unit(int32_t n_units) {
+    const int32_t gidx = blockIdx.x;
+    const int32_t xcd = gidx & 7, slot = gidx >> 3;
+    const int32_t per = n_units >> 3, rem = n_units & 7;
+    return (int64_t)xcd * per + (xcd < rem ? xcd : rem) + slot;
+}
+
+/* Two 64 x 32-variant tiles of tile_of_f4 per workgroup, each as two
+ * 32 x 32-variant wave tiles, two waves per SIMD.  The products come from
+ * v_mfma_scale_f32_32x32x64_f8f6f4, which issues half as many instructions
+ * per product as the 16x16x128 form; the two waves of a SIMD hide each
+ * other's VALU work (fragment ANDs, the filter), which within one wave runs
+ * between its MFMAs rather than beside them.
+ *
+ * Operands come from the grouped 3-step layout (k_class_f4<3, true>): the
+ * 32 variants of a group, one plane and one 16-byte column of a super-step
+ * are 512 contiguous bytes, so every fragment load of a wave reads 1 KB
+ * contiguous (lane i + 32h: row i, column h + 2cp, the K half of the
+ * 32x32x64 operand) straight into registers: no LDS staging and no barrier
+ * in the K loop.  Per super-step and column pair cp, three 64-sample K
+ * chunks (nibble bits 0-2): A block p is plane p of the wave's 32 A
+ * variants, B block q plane q of the 32 B variants, 9 MFMAs per chunk;
+ * after a column pair's chunks its six registers are reloaded with the next
+ * super-step's.  Lane l holds, in accumulator (p, q) element 4j + r, A
+ * variant 8j + 4(l / 32) + r of the wave's group against B variant l % 32,
+ * so every lane has all 9 products of its 16 variant pairs.
+ *
+ * Classes follow each other in the K order (class 0 from super-step 0,
+ * class 1 from f4_super3(ks0)); the accumulators run on across the class
+ * boundary, where the class-0 counts (below 2^16: GEPI_MAX_SAMPLES) go to
+ * LDS, two per dword; class 1 is
+ * the final accumulator less the class-0 count.  Padding K steps of a
+ * class's last super-step are zero in the layout and add nothing.  The
+ * filter runs per lane on its 16 pairs; flagged pairs are finished in the
+ * lane (no parking list: the LDS holds the class-0 counts).               */
+template <int FFUN, bool F32>
+__global__ void __launch_bounds__(F4W_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_count_f4w(CountArgs g) {
+    __shared__ uint32_t c0lds[4][F4W_PARK * 64];
+    __shared__ PendStage stage;
+    stage_init(stage);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t tile = g.tile_begin + 2 * band_unit((g.n_tiles + 1) >> 1) + (wave >> 1);
+    const bool live = tile < g.tile_begin + g.n_tiles;
+    int32_t ta = 0, tb = 0;
+    if (live) tile_of_f4(tile, g.n_var_pad / TM, ta, tb);
+    const int32_t gA = 2 * ta + (wave & 1), gB = tb;
+    const int32_t u1 = f4_super3(g.ks0), n_u = u1 + f4_super3(g.n_ks - g.ks0);
+    const int32_t last = n_u > 0 ? n_u - 1 : 0; /* n_u == 0: super-step 0 exists */
+    const char *lay = reinterpret_cast<const char *>(g.planes_k);
+    const int64_t ustride = (int64_t)g.n_var_pad * 192; /* bytes per super-step */
+    const int loff = (lane >> 5) * 512 + (lane & 31) * 16;
+    uint32_t *P = c0lds[wave] + lane; /* dword d of this lane at P[64 d] */
+
+    /* c[cp][p]: A plane p, column h + 2cp; c[cp][3 + q]: B plane q */
+    uint4 c[2][6];
+    auto fetch = [&](int32_t u, int cp) {
+        const char *base = lay + (int64_t)u * ustride;
+        const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char *>(base + (int64_t)gA * 6144), (short)0, 6144, F4_RSRC_WORD3);
+        const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char *>(base + (int64_t)gB * 6144), (short)0, 6144, F4_RSRC_WORD3);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            c[cp][p] = as_u4(__builtin_amdgcn_raw_buffer_load_b128(ra, loff + p * 2048 + cp * 1024, 0, 0));
+            c[cp][3 + p] =
+                as_u4(__builtin_amdgcn_raw_buffer_load_b128(rb, loff + p * 2048 + cp * 1024, 0, 0));
+        }
+    };
+    v16f acc[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[p][q][e] = 0.0f;
+    auto chunks = [&](int cp) {
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const uint32_t m = 0x11111111u << b;
+            v8i fb[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) fb[q] = frag_f4(c[cp][3 + q], m);
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                const v8i fa = frag_f4(c[cp][p], m);
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    acc[p][q] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                        fa, fb[q], acc[p][q], F4_FMT, F4_FMT, 0, f4_scale(b), 0, f4_scale(b));
+            }
+        }
+    };
+    auto sstep = [&](int32_t u) {
+        const int32_t un = u + 1 < last ? u + 1 : last;
+        chunks(0);
+        fetch(un, 0);
+        chunks(1);
+        fetch(un, 1);
+    };
+    const int32_t nu_live = live ? n_u : 0;
+    const int32_t u1_live = live ? u1 : 0;
+    if (live) {
+        fetch(0, 0);
+        fetch(0, 1);
+    }
+    int32_t u = 0;
+    for (; u < u1_live; u++) sstep(u);
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int e = 0; e < 16; e += 2)
+                P[((p * 3 + q) * 8 + e / 2) * 64] =
+                    (uint32_t)acc[p][q][e] | ((uint32_t)acc[p][q][e + 1] << 16);
+    for (; u < nu_live; u++) sstep(u);
+
+    const FilterConsts fc = filter_consts(g.st, g.n0, g.n1);
+    __syncthreads(); /* the stage counter is cleared */
+    const int32_t B = gB * 32 + (lane & 31);
+    const int32_t Ab = gA * 32 + 4 * (lane >> 5);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int32_t A = Ab + 8 * (e >> 2) + (e & 3);
+        if (!live || A >= g.n_var || B > A) continue;
+        if constexpr (F32) {
+            float cn[2][3][3];
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const uint32_t w = P[((p * 3 + q) * 8 + e / 2) * 64];
+                    cn[0][p][q] = (float)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
+                    cn[1][p][q] = acc[p][q][e] - cn[0][p][q];
+                }
+            filter_pair<FFUN, false>(g, stage, fc, A, B, cn);
+        } else {
+            int32_t cn[2][3][3];
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const uint32_t w = P[((p * 3 + q) * 8 + e / 2) * 64];
+                    cn[0][p][q] = (int32_t)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
+                    cn[1][p][q] = (int32_t)acc[p][q][e] - cn[0][p][q];
+                }
+            filter_pair<FFUN, false>(g, stage, fc, A, B, cn);
+        }
+    }
+    stage_flush(g, stage);
+}
+
 } // namespace gepi
 
 #endif

@@ -197,7 +197,7 @@ int validate_params(const gepi_params *par) {
     if (par->topmost < 0) return gepi_fail(GEPI_ERR_ARG, "topmost must be >= 0");
     if (par->ffun != GEPI_FFUNC_ENTROPY && par->ffun != GEPI_FFUNC_FISHERPV)
         return gepi_fail(GEPI_ERR_ARG, "unknown objective %d", par->ffun);
-    if (par->count_path < 0 || par->count_path > 3)
+    if (par->count_path < 0 || par->count_path > 4)
         return gepi_fail(GEPI_ERR_ARG, "unknown count path %d", par->count_path);
     return GEPI_OK;
 }
@@ -327,7 +327,7 @@ int bind_scratch(gepi_panel *p, ScanBufs &b, int32_t tmax, size_t cw_bytes, size
  * tiles).                                                                  */
 constexpr unsigned long long WG_SELECT_MAX = 1ull << 18;
 inline bool wg_select(int path, unsigned long long count) {
-    return path == 3 && count <= WG_SELECT_MAX;
+    return (path == 3 || path == 4) && count <= WG_SELECT_MAX;
 }
 
 TopkDesc topk_desc(const ScanBufs &b, int cur, unsigned long long cap, int64_t k,
@@ -398,8 +398,12 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             cls0_high = big0 ? 1 : 0;
     }
     int path = par->count_path;
-    if (path == 0) path = 3;
-    if (noalloc && path != 3)
+    /* fp4: wave tiles for entropy; Fisher flags about one variant pair in 8
+     * and the row-staged kernel parks those in LDS, which the wave-tile
+     * kernel's LDS (the class-0 counts) leaves no room for */
+    if (path == 0) path = par->ffun == GEPI_FFUNC_ENTROPY ? 3 : 4;
+    const bool fp4 = path == 3 || path == 4;
+    if (noalloc && !fp4)
         return gepi_fail(GEPI_ERR_ARG, "gepi_scan_noalloc runs the fp4 count path only (path %d "
                                        "asked); use gepi_scan", path);
     if (path == 2 && cls0_high < 0)
@@ -427,22 +431,23 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     const int32_t tv = path == 2 ? TM : TV;
     const int64_t nt_side = n_var_pad / tv;
     const int64_t n_a4 = n_var_pad / TM;
-    const int64_t n_tiles = path == 3 ? n_a4 * (n_a4 + 1) : nt_side * (nt_side + 1) / 2;
-    const int64_t pairs_per_tile = path == 3 ? (int64_t)TM * TB4 : (int64_t)tv * tv;
-    const int32_t n_ks = path == 3 ? wused / 4 : wused / 2;
+    const int64_t n_tiles = fp4 ? n_a4 * (n_a4 + 1) : nt_side * (nt_side + 1) / 2;
+    const int64_t pairs_per_tile = fp4 ? (int64_t)TM * TB4 : (int64_t)tv * tv;
+    const int32_t n_ks = fp4 ? wused / 4 : wused / 2;
 
     /* int8: n_ks 64-sample words per plane row; fp4: n_u 64-byte super-step
      * rows, then the rows' fourth-K-step set bits ([2][n_var_pad * 3] int32) */
-    const int32_t n_u = f4_super(w0 / 4) + f4_super(w1 / 4);
+    const int32_t n_u = path == 3 ? f4_super3(w0 / 4) + f4_super3(w1 / 4)
+                                  : f4_super(w0 / 4) + f4_super(w1 / 4);
     const size_t f4_lay = (size_t)std::max(n_u, 1) * n_var_pad * 3 * 64;
-    const size_t cls_bytes = path != 3 ? (size_t)n_var_pad * 3 * wc * 4 : 0;
+    const size_t cls_bytes = !fp4 ? (size_t)n_var_pad * 3 * wc * 4 : 0;
     const size_t clsk_bytes =
-        path == 3 ? f4_lay + (size_t)2 * n_var_pad * 3 * sizeof(int32_t)
+        fp4 ? f4_lay + (size_t)2 * n_var_pad * 3 * sizeof(int32_t)
                   : path == 2 ? (size_t)std::max(n_ks, 1) * n_var_pad * 3 * 8 : 0;
     /* fp4 path: per 32-sample word of the panel, the compression of each
      * class (mask bits at or beyond n_samples are dropped) */
     std::vector<CompressWord> cw;
-    if (path == 3) {
+    if (fp4) {
         int32_t c0n, c1n;
         build_compress_words(mask_ctrl, mask_case, p->n_samples, p->w64, cw, c0n, c1n);
     }
@@ -484,11 +489,11 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         if (clsk_bytes) grown(p->d_clsk, p->clsk_bytes, clsk_bytes);
         if (clsk_bytes && (rc = grow(&p->d_clsk, &p->clsk_bytes, clsk_bytes)) != GEPI_OK)
             return rc;
-        if (path != 3) grown(p->d_perm, p->perm_bytes, perm.size() * 4);
-        if (path != 3 && (rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK)
+        if (!fp4) grown(p->d_perm, p->perm_bytes, perm.size() * 4);
+        if (!fp4 && (rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK)
             return rc;
-        if (path == 3) grown(p->d_cw, p->cw_bytes, cw.size() * sizeof(CompressWord));
-        if (path == 3 &&
+        if (fp4) grown(p->d_cw, p->cw_bytes, cw.size() * sizeof(CompressWord));
+        if (fp4 &&
             (rc = grow(&p->d_cw, &p->cw_bytes, cw.size() * sizeof(CompressWord))) != GEPI_OK)
             return rc;
         if (cap > p->buf_cap) alloc += (int64_t)(2 * sizeof(Cand) * cap);
@@ -522,21 +527,27 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         b.llo = p->tabs.llo;
         b.lfact = p->tabs.lfact;
     }
-    int32_t *d_pc3 = path == 3 ? reinterpret_cast<int32_t *>(
+    int32_t *d_pc3 = fp4 ? reinterpret_cast<int32_t *>(
                                      reinterpret_cast<char *>(b.clsk) + f4_lay)
                                : nullptr;
 
-    if (path == 3)
+    if (fp4)
         HIP_TRY(hipMemcpyAsync(b.cw, cw.data(), cw.size() * sizeof(CompressWord),
                                hipMemcpyHostToDevice, p->stream));
     else
         HIP_TRY(hipMemcpyAsync(b.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice,
                                p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
-    if (path == 3) {
+    if (fp4) {
         const int64_t rows = n_var_pad * 3;
-        if (n_ks > 0) {
-            k_class_f4<4><<<(unsigned)((rows + 255) / 256), 256, 0, p->stream>>>(
+        const unsigned cgrid = (unsigned)((rows + 255) / 256);
+        if (n_ks > 0 && path == 3) {
+            k_class_f4<3, true><<<cgrid, 256, 0, p->stream>>>(
+                p->d_nat, (int32_t)n_var, p->w64, b.cw, rows, f4_super3(w0 / 4),
+                reinterpret_cast<uint4 *>(b.clsk), d_pc3);
+            HIP_TRY(hipGetLastError());
+        } else if (n_ks > 0) {
+            k_class_f4<4><<<cgrid, 256, 0, p->stream>>>(
                 p->d_nat, (int32_t)n_var, p->w64, b.cw, rows, f4_super(w0 / 4),
                 reinterpret_cast<uint4 *>(b.clsk), d_pc3);
             HIP_TRY(hipGetLastError());
@@ -580,7 +591,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ca.w0 = w0;
     ca.wused = wused;
     ca.n_ks = n_ks;
-    ca.ks0 = path == 3 ? w0 / 4 : w0 / 2;
+    ca.ks0 = fp4 ? w0 / 4 : w0 / 2;
     ca.cls0_high = cls0_high > 0 ? 1 : 0;
     ca.f32 = f32_counts(n0, n1) ? 1 : 0;
     ca.n0 = n0;
@@ -628,8 +639,17 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         ea.cand = b.cand[cur];
         k_reset_pending<<<1, 1, 0, p->stream>>>(b.state);
         HIP_TRY(hipEventRecord(p->ev0, p->stream));
-        const unsigned grid = (unsigned)(t1 - t0);
+        const unsigned grid = path == 3 ? (unsigned)((t1 - t0 + 1) / 2) : (unsigned)(t1 - t0);
         if (path == 3) {
+            if (entropy && f32)
+                k_count_f4w<GEPI_FFUNC_ENTROPY, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+            else if (entropy)
+                k_count_f4w<GEPI_FFUNC_ENTROPY, false><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+            else if (f32)
+                k_count_f4w<GEPI_FFUNC_FISHERPV, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+            else
+                k_count_f4w<GEPI_FFUNC_FISHERPV, false><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+        } else if (path == 4) {
             if (entropy && f32)
                 k_count_f4<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, p->stream>>>(ca);
             else if (entropy)
@@ -688,7 +708,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
                 HIP_TRY(hipMemcpyAsync(b.state, &hs, sizeof hs, hipMemcpyHostToDevice, p->stream));
                 launch_select(p, b, b.cand[cur], (uint32_t)k, kept);
                 k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
-                                                       t0, t1, tv, path == 3 ? (int32_t)n_a4 : 0);
+                                                       t0, t1, tv, fp4 ? (int32_t)n_a4 : 0);
                 k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
                 cur ^= 1;
             }
@@ -707,7 +727,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             } else {
                 launch_select(p, b, b.cand[cur], (uint32_t)k, hs.count);
                 k_compact<<<4096, 256, 0, p->stream>>>(b.cand[cur], b.state, b.cand[cur ^ 1],
-                                                       0, 0, tv, path == 3 ? (int32_t)n_a4 : 0);
+                                                       0, 0, tv, fp4 ? (int32_t)n_a4 : 0);
                 k_finalize<<<1, 1, 0, p->stream>>>(b.state, par->ffun);
                 cur ^= 1;
             }

@@ -21,7 +21,8 @@
  *                  128-sample K steps and written four K steps at a time as
  *                  64-byte super-step rows (class 0, then class 1 from the
  *                  next super-step), with each row's set bits in the fourth
- *                  K steps.
+ *                  K steps; or three K steps at a time in the grouped
+ *                  layout of the wave-tile count kernel.
  * The QA kernels replace the loops of checkQuality (src/reader/
  * PlinkReader.h:237-335), which count missing calls on the host.
  */
@@ -179,11 +180,15 @@ __device__ __forceinline__ uint4 super_group(uint32_t a, uint32_t b, uint32_t c,
 /* A stream of bits appended in order, cut into 128-sample K steps, and
  * written S (3 or 4) K steps at a time as super-step rows
  * out[(u * rows + r) * 4 + g]; with S = 4, pc3 counts the set bits of every
- * fourth K step (the count kernel's correction of that step).  Every thread
+ * fourth K step (the count kernel's correction of that step).  GRP: the
+ * grouped layout of the wave-tile kernel instead, uint4 g of row r = 3v + p
+ * at out[(((u * NG + v / 32) * 3 + p) * 4 + g) * 32 + v % 32] (NG = rows /
+ * 96 groups of 32 variants), so that the 32 variants of a group, one plane
+ * and one 16-byte column, are 512 contiguous bytes.  Every thread
  * of a launch appends the same counts in the same order, so the bookkeeping
  * is uniform.  The completed K steps of the current super-step advance
  * through k0, k1, k2 (an indexed array would be placed in scratch memory). */
-template <int S>
+template <int S, bool GRP = false>
 struct SuperStream {
     uint64_t acc;
     int32_t na, nq, np, pc3;
@@ -209,11 +214,21 @@ struct SuperStream {
             /* S = 3: kr is the third K step, bit 3 of every nibble stays 0 */
             const uint4 c = S == 4 ? k2 : kr, e = S == 4 ? kr : make_uint4(0u, 0u, 0u, 0u);
             if (r < rows) {
-                uint4 *dst = out + (u * rows + r) * 4;
+                /* GRP: uint4 g of the row is 32 uint4 after uint4 g - 1 */
+                int64_t at, gs;
+                if (GRP) {
+                    const int64_t v = r / 3, pl = r - 3 * v;
+                    at = (((u * (rows / 96) + (v >> 5)) * 3 + pl) * 4) * 32 + (v & 31);
+                    gs = 32;
+                } else {
+                    at = (u * rows + r) * 4;
+                    gs = 1;
+                }
+                uint4 *dst = out + at;
                 dst[0] = super_group(k0.x, k1.x, c.x, e.x);
-                dst[1] = super_group(k0.y, k1.y, c.y, e.y);
-                dst[2] = super_group(k0.z, k1.z, c.z, e.z);
-                dst[3] = super_group(k0.w, k1.w, c.w, e.w);
+                dst[gs] = super_group(k0.y, k1.y, c.y, e.y);
+                dst[2 * gs] = super_group(k0.z, k1.z, c.z, e.z);
+                dst[3 * gs] = super_group(k0.w, k1.w, c.w, e.w);
             }
             pc3 += __popc(e.x) + __popc(e.y) + __popc(e.z) + __popc(e.w);
             u++;
@@ -258,10 +273,11 @@ struct SuperStream {
  * scans, 3: the batch engine).  Class 0 fills super-steps [0, u1) (u1 =
  * f4_super or f4_super3 of its K steps), class 1 the super-steps from u1
  * on.  S = 4: pc3[c * rows + r] = set bits of row r in the fourth K steps
- * of class c's super-steps.  The panel rows of a workgroup are staged
+ * of class c's super-steps.  GRP: the grouped layout (SuperStream; rows a
+ * multiple of 96).  The panel rows of a workgroup are staged
  * through LDS 8 words at a time (coalesced reads).                         */
 constexpr int CLS_CHUNK = 8;
-template <int S>
+template <int S, bool GRP = false>
 __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ nat, int32_t n_var,
                                                    int32_t w64, const CompressWord *__restrict__ cw,
                                                    int64_t rows, int32_t u1,
@@ -271,7 +287,7 @@ __global__ void __launch_bounds__(256) k_class_f4(const uint64_t *__restrict__ n
     const int64_t r0 = (int64_t)blockIdx.x * 256;
     const int64_t r = r0 + threadIdx.x;
     const int64_t real_rows = (int64_t)n_var * 3;
-    SuperStream<S> s0, s1;
+    SuperStream<S, GRP> s0, s1;
     s0.init(0);
     s1.init(u1);
     const CompressWord *c0 = cw, *c1 = cw + 2 * w64;

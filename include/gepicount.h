@@ -70,8 +70,10 @@ typedef struct gepi_params {
     int32_t gpu_num;       /* -G: HIP device ordinal                           */
     int32_t ffun;          /* -f: gepi_ffunc                                   */
     int64_t topmost;       /* -N: results to keep                              */
-    int32_t count_path;    /* 0 = fastest (fp4 MFMA), 1 = popcount (VALU),     
-                            * 2 = int8 MFMA, 3 = fp4 (e2m1) MFMA                */
+    int32_t count_path;    /* 0 = fastest (3 for entropy, 4 for Fisher),
+                            * 1 = popcount (VALU), 2 = int8 MFMA, 3 = fp4
+                            * (e2m1) MFMA on 32 x 32 wave tiles, 4 = fp4 MFMA
+                            * on LDS-staged 64 x 32 tiles                    */
     int32_t intermediate;  /* -D: report progress and intermediate results
                             * through the thread's progress hook              */
     int32_t blk_x, blk_y;  /* --block-sizes: accepted for compatibility        */
@@ -159,10 +161,10 @@ int gepi_scan(gepi_panel *panel, const uint64_t *mask_ctrl, const uint64_t *mask
  * under HIP graph capture, allocation-free serving loops).  Every buffer a
  * scan needs (score tables, scan state, class layout, filter survivors,
  * candidate lists) lives in one fixed array of GEPI_NOALLOC_SCRATCH_MIB MiB
- * per GPU, held until the process ends.  Fp4 count path only (count_path 0
- * or 3); a scan whose buffers do not fit fails with GEPI_ERR_LIMIT (with
- * top-k up to 100,000: about 60,000 variants at 2,048 samples, 15,000 at
- * 8,192).
+ * per GPU, held until the process ends.  Fp4 count paths only (count_path
+ * 0, 3 or 4); a scan whose buffers do not fit fails with GEPI_ERR_LIMIT
+ * (with top-k up to 100,000: about 43,000 variants at 2,048 samples and
+ * 12,000 at 8,192 on path 3, 52,000 and 15,000 on path 4).
  * Results equal gepi_scan's.  Calls on one GPU share the array: calls that
  * overlap on one GPU (from several threads) race and must not be made.      */
 #define GEPI_NOALLOC_SCRATCH_MIB 64

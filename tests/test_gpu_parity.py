@@ -59,7 +59,7 @@ def random_panel(seed, n_var, n, miss, cls_p=(0.5, 0.45, 0.05)):
     return bits, valid, m0, m1
 
 
-PATHS = [1, 2, 3]  # 1 = popcount (VALU), 2 = int8 MFMA, 3 = fp4 MFMA
+PATHS = [1, 2, 3, 4]  # popcount (VALU), int8 MFMA, fp4 wave tiles, fp4 LDS-staged tiles
 
 
 @pytest.mark.parametrize("path", PATHS)
@@ -105,38 +105,42 @@ def test_fisher_objective(seed, n_var, n, miss, k, path):
     assert_same(got, ref, "fisher")
 
 
+@pytest.mark.parametrize("path", [3, 4])
 @pytest.mark.parametrize("ffun", [0, 1])
 @pytest.mark.parametrize("n0,n1", [(4096, 4096), (4200, 4200), (1100, 15000), (1000, 16000)])
-def test_fp4_filter_count_types(n0, n1, ffun):
+def test_fp4_filter_count_types(n0, n1, ffun, path):
     """The fp4 path filters on its fp32 accumulators when every product of a
     class-0 and a class-1 count is exact in float (n0 n1 <= 2^24; 4096 x 4096
     is the largest balanced case, where products reach 2^24) and the samples
     number at most 16,384 (1100 + 15000 is near both limits), and on int32
     conversions otherwise (4200 x 4200; 1000 + 16000 samples).  Both must
     admit every evaluation the exact top-k needs."""
-    print("case fp4 filter n0=%d n1=%d ffun=%d" % (n0, n1, ffun), flush=True)
+    print("case fp4 filter n0=%d n1=%d ffun=%d path=%d" % (n0, n1, ffun, path), flush=True)
     rng = np.random.default_rng(83)
     n = n0 + n1
     codes = orc.random_codes(rng, 48, n, 0.02)
     cls = rng.permutation(np.r_[np.zeros(n0, np.int64), np.ones(n1, np.int64)])
     bits, valid = orc.planes_from_codes(codes)
     m0, m1 = orc.masks_from_classes(cls)
-    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=300, ffun=ffun, count_path=3)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=300, ffun=ffun, count_path=path)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), 300, ffun=ffun)
-    assert st["n_class0"] == n0 and st["n_class1"] == n1
+    assert st["n_class0"] == n0 and st["n_class1"] == n1 and st["count_path"] == path
     assert_same(got, ref, "fp4-filter")
 
 
+@pytest.mark.parametrize("path", [3, 4])
 @pytest.mark.parametrize("n0,n1", [(512, 1024), (640, 384), (768, 1), (1, 896), (1024, 0),
-                                   (500, 1020), (130, 4), (3, 2)])
-def test_fp4_sign_step_residues(n0, n1):
-    """A super-step of the fp4 layout holds 4 K steps of 128 samples, the
-    fourth in the e2m1 sign bit with a per-row correction; each class
-    starts a new super-step.  Class sizes cover 0, 1, 2 and 3 K steps past
-    full super-steps in either class, exact multiples of 512 samples, one
-    or no sample in a class, and partial last K steps.  Every evaluation
-    must equal the oracle's."""
-    print("case fp4 sign step n0=%d n1=%d" % (n0, n1), flush=True)
+                                   (500, 1020), (130, 4), (3, 2), (384, 1152), (385, 383)])
+def test_fp4_super_step_residues(n0, n1, path):
+    """A super-step of the LDS-staged fp4 layout (path 4) holds 4 K steps
+    of 128 samples, the fourth in the e2m1 sign bit with a per-row
+    correction; one of the wave-tile layout (path 3) holds 3, and its
+    accumulators run across the class boundary.  Each class starts a new
+    super-step.  Class sizes cover 0, 1, 2 and 3 K steps past full
+    super-steps of either layout in either class, exact multiples of 384
+    and 512 samples, one or no sample in a class, and partial last K steps.
+    Every evaluation must equal the oracle's."""
+    print("case fp4 super-step n0=%d n1=%d path=%d" % (n0, n1, path), flush=True)
     rng = np.random.default_rng(91 + n0 + 7 * n1)
     n = n0 + n1
     codes = orc.random_codes(rng, 21, n, 0.03)
@@ -145,15 +149,15 @@ def test_fp4_sign_step_residues(n0, n1):
     m0, m1 = orc.masks_from_classes(cls)
     R = bits.shape[0]
     k = 5 * R * (R + 1) // 2
-    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k, count_path=3)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k, count_path=path)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), k)
-    assert st["n_class0"] == n0 and st["n_class1"] == n1
-    assert_same(got, ref, "fp4 sign step")
+    assert st["n_class0"] == n0 and st["n_class1"] == n1 and st["count_path"] == path
+    assert_same(got, ref, "fp4 super-step")
 
 
 def test_fisher_parked_pairs_overflow(monkeypatch):
-    """The fp4 single-scan kernel parks variant pairs that pass the
-    chi-square bound in LDS and finishes them (function bits, Fisher's exact
+    """The LDS-staged fp4 kernel (path 4, Fisher's default) parks variant
+    pairs that pass the chi-square bound in LDS and finishes them (function bits, Fisher's exact
     check) one per thread after the tile; with a loose threshold (large k,
     two-tile bands) a tile flags more pairs than the list holds and the rest
     are finished in the lane.  Together they must keep exactly the oracle's
@@ -163,8 +167,9 @@ def test_fisher_parked_pairs_overflow(monkeypatch):
     k = 50000
     monkeypatch.setenv("GEPI_BAND_TILES", "2")
     got, st = ge.process_gpu_1(bits, valid, m0, m1, 400, topmost=k,
-                               ffun=ge.GEPI_FFUNC_FISHERPV, count_path=3)
+                               ffun=ge.GEPI_FFUNC_FISHERPV)
     monkeypatch.delenv("GEPI_BAND_TILES")
+    assert st["count_path"] == 4
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 400), k, ffun=1)
     assert st["bands"] > 2
     assert_same(got, ref, "fisher-parked")
@@ -304,23 +309,25 @@ def test_candidate_overflow_recovery(monkeypatch, path):
     assert_same(got, ref, "candidate-overflow")
 
 
+@pytest.mark.parametrize("path", [3, 4])
 @pytest.mark.parametrize("cand_cap", [None, "1"])
-def test_fp4_tile_panels_and_band_edges(monkeypatch, cand_cap):
+def test_fp4_tile_panels_and_band_edges(monkeypatch, cand_cap, path):
     """fp4 tiles are numbered in panels of 8 A rows (a shared rectangle
     column by column, then the rows' remaining tiles); 620 variants give 10 A
     rows, a full panel and a partial one of 2.  Bands of 7 tiles straddle
     panel and row boundaries; with the candidate buffer forced to overflow,
     the records of a band are dropped through the inverse of that numbering
-    and the band is redone.  Lost or duplicated tiles would break the exact
-    match."""
-    print("case fp4 panels cand_cap=%s" % cand_cap, flush=True)
+    and the band is redone.  The wave-tile kernel (path 3) takes two tiles
+    per workgroup, so a band of 7 ends in a workgroup with one tile.  Lost
+    or duplicated tiles would break the exact match."""
+    print("case fp4 panels cand_cap=%s path=%d" % (cand_cap, path), flush=True)
     bits, valid, m0, m1 = random_panel(61, 620, 200, 0.02)
     k = 6000
     monkeypatch.setenv("GEPI_BAND_TILES", "7")
     if cand_cap:
         monkeypatch.setenv("GEPI_CAND_CAP", cand_cap)
         monkeypatch.setenv("GEPI_PEND_CAP", "4000000")
-    got, st = ge.process_gpu_1(bits, valid, m0, m1, 200, topmost=k, count_path=3)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, 200, topmost=k, count_path=path)
     for v in ("GEPI_BAND_TILES", "GEPI_CAND_CAP", "GEPI_PEND_CAP"):
         monkeypatch.delenv(v, raising=False)
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 200), k)

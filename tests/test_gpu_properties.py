@@ -141,7 +141,7 @@ def test_large_entropy_top10000_equals_gold(large):
 def test_large_count_paths_agree(large):
     print("case large paths agree", flush=True)
     ref, _ = large["panel"].scan(large["m0"], large["m1"], topmost=10000, count_path=3)
-    for path in (1, 2):
+    for path in (1, 2, 4):
         got, st = large["panel"].scan(large["m0"], large["m1"], topmost=10000, count_path=path)
         assert st["count_path"] == path
         for f in ("sA", "sB", "fun", "w", "x", "y", "z"):
@@ -151,7 +151,15 @@ def test_large_count_paths_agree(large):
 
 def test_large_fisher_top1000(large):
     print("case large Fisher top-1000", flush=True)
-    res, _ = large["panel"].scan(large["m0"], large["m1"], topmost=1000,
-                                 ffun=ge.GEPI_FFUNC_FISHERPV)
+    res, st = large["panel"].scan(large["m0"], large["m1"], topmost=1000,
+                                  ffun=ge.GEPI_FFUNC_FISHERPV)
+    assert st["count_path"] == 4
     check_records(large, res, 1000, 1)
     check_subset_completeness(large, res, 1000, 1, n_top_vars=500, n_rand=300)
+    # the wave-tile kernel (finishing flagged pairs in the lane) agrees
+    got, st = large["panel"].scan(large["m0"], large["m1"], topmost=1000,
+                                  ffun=ge.GEPI_FFUNC_FISHERPV, count_path=3)
+    assert st["count_path"] == 3
+    for f in ("sA", "sB", "fun", "w", "x", "y", "z"):
+        np.testing.assert_array_equal(got[f], res[f])
+    np.testing.assert_array_equal(got["obj"].view(np.uint64), res["obj"].view(np.uint64))
