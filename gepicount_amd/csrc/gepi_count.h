@@ -891,6 +891,12 @@ constexpr int F4W_THREADS = 256; /* 4 waves: two 64 x 32-variant tiles, 32 x 32 
 
 /* Class-0 counts of a wave tile, two 16-bit counts per dword (in LDS). */
 constexpr int F4W_PARK = 72;
+
+/* The automatic choice takes the wave tiles for scans of at least this many
+ * 128-sample K steps (both classes): measured against the LDS-staged kernel
+ * at 20,000 variants, 0.99-1.09x its time up to 1,024 samples, 0.88-0.96x
+ * from 1,536 on. */
+constexpr int32_t F4W_MIN_KSTEPS = 10;
 static_assert(GEPI_MAX_SAMPLES <= 65535, "a class count must fit 16 bits");
 
 /* Workgroup -> its two tiles (2u, 2u + 1 of the band), units numbered as

@@ -398,10 +398,15 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             cls0_high = big0 ? 1 : 0;
     }
     int path = par->count_path;
-    /* fp4: wave tiles for entropy; Fisher flags about one variant pair in 8
-     * and the row-staged kernel parks those in LDS, which the wave-tile
+    /* fp4: wave tiles for entropy from F4W_MIN_KSTEPS K steps on (below, a
+     * class's padding K steps, multiplied on the 3-step layout, cost more
+     * than the wave tiles save); Fisher flags about one variant pair in 8
+     * and the LDS-staged kernel parks those in LDS, which the wave-tile
      * kernel's LDS (the class-0 counts) leaves no room for */
-    if (path == 0) path = par->ffun == GEPI_FFUNC_ENTROPY ? 3 : 4;
+    if (path == 0) {
+        const int32_t ks = (n0 + 127) / 128 + (n1 + 127) / 128;
+        path = par->ffun == GEPI_FFUNC_ENTROPY && ks >= F4W_MIN_KSTEPS ? 3 : 4;
+    }
     const bool fp4 = path == 3 || path == 4;
     if (noalloc && !fp4)
         return gepi_fail(GEPI_ERR_ARG, "gepi_scan_noalloc runs the fp4 count path only (path %d "

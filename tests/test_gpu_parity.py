@@ -428,3 +428,23 @@ def test_cli_report_equals_oracle_report():
                           capture_output=True, text=True, timeout=120)
     assert report == gold.stderr.splitlines()[:102]
     assert "Cleaned" in r.stdout
+
+
+@pytest.mark.parametrize("n0,n1,ffun,path", [(390, 310, 0, 4), (640, 640, 0, 3), (640, 512, 0, 4),
+                                              (1200, 1200, 1, 4), (1200, 1200, 0, 3)])
+def test_automatic_count_path(n0, n1, ffun, path):
+    """count_path 0 takes the fp4 wave tiles (3) for entropy scans of at
+    least 10 K steps of 128 samples (both classes, each rounded up) and the
+    LDS-staged fp4 kernel (4) otherwise and for Fisher; either way the
+    result equals the oracle's."""
+    print("case auto path n0=%d n1=%d ffun=%d" % (n0, n1, ffun), flush=True)
+    rng = np.random.default_rng(n0 + 3 * n1 + ffun)
+    n = n0 + n1
+    codes = orc.random_codes(rng, 30, n, 0.02)
+    cls = rng.permutation(np.r_[np.zeros(n0, np.int64), np.ones(n1, np.int64)])
+    bits, valid = orc.planes_from_codes(codes)
+    m0, m1 = orc.masks_from_classes(cls)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=500, ffun=ffun)
+    assert st["count_path"] == path
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), 500, ffun=ffun)
+    assert_same(got, ref, "auto path")

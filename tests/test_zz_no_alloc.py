@@ -50,8 +50,10 @@ def test_noalloc_equals_scan_on_consecutive_calls():
         m0, m1 = synth.masks_from_pheno(p["pheno"])
         with ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"]) as h:
             got, st = h.scan_noalloc(m0, m1, topmost=k, ffun=ffun)
-            # the fp4 kernel chosen by objective: wave tiles for entropy
-            assert st["count_path"] == (3 if ffun == 0 else 4), st
+            # the fp4 kernel chosen by objective and size: wave tiles for
+            # entropy scans of at least 10 K steps of 128 samples
+            ks = -(-st["n_class0"] // 128) - (-st["n_class1"] // 128)
+            assert st["count_path"] == (3 if ffun == 0 and ks >= 10 else 4), st
             assert st["bytes_allocated"] == 0, st
             ref, st_ref = h.scan(m0, m1, topmost=k, ffun=ffun)
             assert st_ref["bytes_allocated"] > 0, "a fresh panel's first scan grows its workspace"
