@@ -576,7 +576,8 @@ __device__ __forceinline__ void count_f4_tile(const CountArgs &g, int64_t tile) 
     }
     if constexpr (PARK) {
         __syncthreads();
-        pair_drain<FFUN>(g, stage, fc, *pairs);
+        pair_drain<FFUN>(g, stage, fc, pairs->rec,
+                         pairs->n < (uint32_t)PAIR_LDS ? pairs->n : (uint32_t)PAIR_LDS);
     }
     stage_flush(g, stage);
 }
@@ -882,6 +883,37 @@ __global__ void __launch_bounds__(F4_THREADS, 2) k_count_f4(CountArgs g) {
     count_f4_tile<FFUN, F32>(g, band_tile(g.tile_begin, g.n_tiles));
 }
 
+/* pair_drain over records in global memory written by other waves of the
+ * workgroup after they waited for their stores (f4_wait_stores) and a
+ * barrier: each record read as three 16-byte buffer loads with the sc0
+ * (workgroup-coherent) policy, which miss the CU's L1 (it may hold the
+ * region's lines from a workgroup that used it earlier) and read the XCD's
+ * L2, where the stores are.                                               */
+template <int FFUN>
+__device__ __forceinline__ void pair_drain_l2(const CountArgs &g, PendStage &ps,
+                                              const FilterConsts &fc, const PairRec *rec,
+                                              uint32_t n, uint32_t cap) {
+    static_assert(sizeof(PairRec) == 48, "a parked pair is three 16-byte loads");
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<PairRec *>(rec), (short)0, (int)(cap * sizeof(PairRec)), 0x00020000);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        u32x4 w[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            w[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(i * 48 + 16 * k), 0, 1);
+        PairRec r;
+        __builtin_memcpy(&r, w, sizeof r);
+        int32_t c[2][3][3];
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) c[k][p][q] = r.c[k][p][q];
+        filter_pair<FFUN, true>(g, ps, fc, r.A, r.B, c);
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* fp4 wave-tile path (single scans)                                      */
 
@@ -891,6 +923,10 @@ constexpr int F4W_THREADS = 256; /* 4 waves: two 64 x 32-variant tiles, 32 x 32 
 
 /* Class-0 counts of a wave tile, two 16-bit counts per dword (in LDS). */
 constexpr int F4W_PARK = 72;
+
+/* Fisher's parked variant pairs per workgroup (two tiles) of the wave-tile
+ * kernel, in a global region the workgroup claims. */
+constexpr uint32_t F4W_PAIR_CAP = 1024;
 
 /* The automatic choice takes the wave tiles for scans of at least this many
  * 128-sample K steps (both classes): measured against the LDS-staged kernel
@@ -934,14 +970,23 @@ __device__ __forceinline__ int64_t band_unit(int32_t n_units) {
  * LDS, two per dword; class 1 is
  * the final accumulator less the class-0 count.  Padding K steps of a
  * class's last super-step are zero in the layout and add nothing.  The
- * filter runs per lane on its 16 pairs; flagged pairs are finished in the
- * lane (no parking list: the LDS holds the class-0 counts).               */
+ * filter runs per lane on its 16 pairs.  Fisher flags about one variant
+ * pair in eight: with g.park those are parked in a global region the
+ * workgroup claims (park_claim; the LDS holds the class-0 counts) and
+ * finished one per thread after the filter (pair_drain); past g.park_cap,
+ * or without regions, in the lane.                                       */
 template <int FFUN, bool F32>
 __global__ void __launch_bounds__(F4W_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_count_f4w(CountArgs g) {
     __shared__ uint32_t c0lds[4][F4W_PARK * 64];
     __shared__ PendStage stage;
+    constexpr bool PARK = FFUN == GEPI_FFUNC_FISHERPV;
+    __shared__ uint32_t n_park, park_slot;
     stage_init(stage);
+    if (PARK && threadIdx.x == 0) {
+        n_park = 0;
+        park_slot = g.park ? park_claim(g.park_busy, g.park_slots) : PARK_NONE;
+    }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t tile = g.tile_begin + 2 * band_unit((g.n_tiles + 1) >> 1) + (wave >> 1);
@@ -1021,7 +1066,10 @@ k_count_f4w(CountArgs g) {
     for (; u < nu_live; u++) sstep(u);
 
     const FilterConsts fc = filter_consts(g.st, g.n0, g.n1);
-    __syncthreads(); /* the stage counter is cleared */
+    __syncthreads(); /* the stage counter is cleared, the park region claimed */
+    const bool parks = PARK && park_slot != PARK_NONE;
+    PairListG pl{&n_park, parks ? g.park + (int64_t)park_slot * g.park_cap : nullptr,
+                 parks ? g.park_cap : 0u};
     const int32_t B = gB * 32 + (lane & 31);
     const int32_t Ab = gA * 32 + 4 * (lane >> 5);
 #pragma unroll
@@ -1038,7 +1086,7 @@ k_count_f4w(CountArgs g) {
                     cn[0][p][q] = (float)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
                     cn[1][p][q] = acc[p][q][e] - cn[0][p][q];
                 }
-            filter_pair<FFUN, false>(g, stage, fc, A, B, cn);
+            filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, cn, &pl);
         } else {
             int32_t cn[2][3][3];
 #pragma unroll
@@ -1049,8 +1097,20 @@ k_count_f4w(CountArgs g) {
                     cn[0][p][q] = (int32_t)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
                     cn[1][p][q] = (int32_t)acc[p][q][e] - cn[0][p][q];
                 }
-            filter_pair<FFUN, false>(g, stage, fc, A, B, cn);
+            filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, cn, &pl);
         }
+    }
+    if constexpr (PARK) {
+        /* The parks are global stores read by other waves after the
+         * barrier, whose workgroup-scope release does not wait for global
+         * stores (it issues s_barrier before s_waitcnt vmcnt(0)): the wait
+         * comes first here, and the drain reads past the CU's L1
+         * (pair_drain_l2); the region belongs to this XCD (park_claim). */
+        f4_wait_stores();
+        __syncthreads();
+        pair_drain_l2<FFUN>(g, stage, fc, pl.rec, n_park < pl.cap_ ? n_park : pl.cap_, pl.cap_);
+        __syncthreads(); /* the region is read: free it */
+        if (threadIdx.x == 0 && parks) atomicExch(&g.park_busy[park_slot], 0u);
     }
     stage_flush(g, stage);
 }

@@ -76,7 +76,37 @@ struct CountArgs {
     Pend *pend;
     unsigned long long pcap;
     const double *lfact;      /* ln t! (Fisher filter)                              */
+    struct PairRec *park;     /* fp4 wave tiles, Fisher: [park_slots][park_cap]
+                               * regions for parked variant pairs (null: none)   */
+    uint32_t *park_busy;      /* [park_slots] claim flags, then 8 claim counters  */
+    uint32_t park_cap, park_slots;
 };
+
+/* The XCD (0-7) this wave runs on, from the hardware register. */
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u; /* HW_REG_XCC_ID */
+}
+
+/* A parking region of CountArgs::park for this workgroup (thread 0): the
+ * first free slot, from a rotating start, among the slots of the XCD it
+ * runs on.  The XCDs' L2 caches are not coherent with one another, so a
+ * region written and read through one XCD's L2 must not pass through
+ * another's within a launch (a region reused across XCDs read stale
+ * records when another XCD's dirty lines were written back over it).
+ * Every claim is released before its workgroup ends and the host keeps
+ * more slots per XCD than workgroups can be resident on one, so a free
+ * slot is found; the probes are bounded all the same (PARK_NONE: no
+ * region, the pairs are finished in the lane).                           */
+constexpr uint32_t PARK_NONE = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t park_claim(uint32_t *busy, uint32_t slots) {
+    const uint32_t x = xcc_id(), per = slots >> 3, base = x * per;
+    uint32_t s = atomicAdd(&busy[slots + x], 1u) % per;
+    for (uint32_t t = 0; t < 4 * per; t++) {
+        if (atomicCAS(&busy[base + s], 0u, 1u) == 0u) return base + s;
+        s = s + 1 == per ? 0u : s + 1;
+    }
+    return PARK_NONE;
+}
 
 __host__ __device__ inline void tile_of(int64_t t, int32_t &ta, int32_t &tb) {
     /* t = ta*(ta+1)/2 + tb, 0 <= tb <= ta */
@@ -404,10 +434,10 @@ __device__ __forceinline__ float yz_of(int32_t y, int32_t z) { return (float)__m
 __device__ __forceinline__ float yz_of(float y, float z) { return y * z; }
 
 struct PairList;
-template <int FFUN, bool ROLLED, bool DEFER = false, typename CT>
+template <int FFUN, bool ROLLED, bool DEFER = false, typename CT, typename PL = PairList>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                             int32_t A, int32_t B, const CT (&c)[2][3][3],
-                                            PairList *pl = nullptr);
+                                            PL *pl = nullptr);
 
 /* Variant pairs with a flagged row pair, parked in LDS: the lanes that
  * flag one are few and scattered over the waves, and finishing a pair (the
@@ -425,12 +455,26 @@ struct PairList {
     PairRec rec[PAIR_LDS];
     uint32_t n;
     uint32_t pad[3];
+    __device__ __forceinline__ uint32_t cap() const { return PAIR_LDS; }
+    __device__ __forceinline__ PairRec &at(uint32_t i) { return rec[i]; }
 };
 
-template <int FFUN, bool ROLLED, bool DEFER, typename CT>
+/* The same list with its records in global memory (a workgroup's own
+ * region, L2-resident while the workgroup runs) and its counter in LDS. */
+struct PairListG {
+    uint32_t *n; /* LDS */
+    PairRec *rec;
+    uint32_t cap_;
+    __device__ __forceinline__ uint32_t cap() const { return cap_; }
+    __device__ __forceinline__ PairRec &at(uint32_t i) { return rec[i]; }
+};
+__device__ __forceinline__ uint32_t park_reserve(PairList *pl) { return atomicAdd(&pl->n, 1u); }
+__device__ __forceinline__ uint32_t park_reserve(PairListG *pl) { return atomicAdd(pl->n, 1u); }
+
+template <int FFUN, bool ROLLED, bool DEFER, typename CT, typename PL>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                             int32_t A, int32_t B, const CT (&c)[2][3][3],
-                                            PairList *pl) {
+                                            PL *pl) {
     const CT y = c[0][2][2], z = c[1][2][2];
     const CT m = y + z;
     const float mf = (float)m;
@@ -490,9 +534,9 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
     }
     if (A == B) rows &= ~2u; /* (2A, 2A+1) is not a row pair */
     if (DEFER && rows) {
-        const uint32_t pos = atomicAdd(&pl->n, 1u);
-        if (pos < (uint32_t)PAIR_LDS) {
-            PairRec &r = pl->rec[pos];
+        const uint32_t pos = park_reserve(pl);
+        if (pos < pl->cap()) {
+            PairRec &r = pl->at(pos);
 #pragma unroll
             for (int k = 0; k < 2; k++)
 #pragma unroll
@@ -533,10 +577,9 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
  * barrier.                                                                 */
 template <int FFUN>
 __device__ __forceinline__ void pair_drain(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
-                                           const PairList &pl) {
-    const uint32_t n = pl.n < (uint32_t)PAIR_LDS ? pl.n : (uint32_t)PAIR_LDS;
+                                           const PairRec *rec, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const PairRec &r = pl.rec[i];
+        const PairRec &r = rec[i];
         int32_t c[2][3][3];
 #pragma unroll
         for (int k = 0; k < 2; k++)
@@ -547,6 +590,10 @@ __device__ __forceinline__ void pair_drain(const CountArgs &g, PendStage &ps, co
         filter_pair<FFUN, true>(g, ps, fc, r.A, r.B, c);
     }
 }
+
+/* Waits until every global store of this wave has been acknowledged by L2
+ * (s_waitcnt vmcnt(0); expcnt and lgkmcnt at their maxima: no wait). */
+__device__ __forceinline__ void f4_wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 } // namespace gepi
 

@@ -59,6 +59,8 @@ struct gepi_panel {
     size_t perm_bytes = 0;
     gepi::CompressWord *d_cw = nullptr; /* fp4 path: class compression words */
     size_t cw_bytes = 0;
+    gepi::PairRec *d_park = nullptr; /* fp4 wave tiles, Fisher: parked pairs */
+    size_t park_bytes = 0;
     gepi::Cand *d_cand[2] = {nullptr, nullptr};
     gepi::Pend *d_pend = nullptr;
     unsigned long long buf_cap = 0;  /* records in each candidate buffer */

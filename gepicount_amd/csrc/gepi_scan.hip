@@ -166,6 +166,7 @@ void panel_free(gepi_panel *p) {
     hipFree(p->d_clsk);
     hipFree(p->d_perm);
     hipFree(p->d_cw);
+    hipFree(p->d_park);
     hipFree(p->d_cand[0]);
     hipFree(p->d_cand[1]);
     hipFree(p->d_pend);
@@ -398,15 +399,23 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             cls0_high = big0 ? 1 : 0;
     }
     int path = par->count_path;
-    /* fp4: wave tiles for entropy from F4W_MIN_KSTEPS K steps on (below, a
-     * class's padding K steps, multiplied on the 3-step layout, cost more
-     * than the wave tiles save); Fisher flags about one variant pair in 8
-     * and the LDS-staged kernel parks those in LDS, which the wave-tile
-     * kernel's LDS (the class-0 counts) leaves no room for */
+    /* fp4: wave tiles from F4W_MIN_KSTEPS K steps on (below, a class's
+     * padding K steps, multiplied on the 3-step layout, cost more than the
+     * wave tiles save).  Fisher flags about one variant pair in 8; the wave
+     * tiles park those in a global list per workgroup, which the
+     * allocation-free scan has no room for, so there the LDS-staged kernel
+     * (parking in LDS) runs Fisher. */
     if (path == 0) {
         const int32_t ks = (n0 + 127) / 128 + (n1 + 127) / 128;
-        path = par->ffun == GEPI_FFUNC_ENTROPY && ks >= F4W_MIN_KSTEPS ? 3 : 4;
+        const bool fisher = par->ffun != GEPI_FFUNC_ENTROPY;
+        path = ks >= F4W_MIN_KSTEPS && !(fisher && noalloc) ? 3 : 4;
     }
+    /* wave tiles, Fisher: parking regions, four per CU (at most two
+     * workgroups of the kernel are resident on a CU), an eighth of them per
+     * XCD */
+    const uint32_t park_slots = (uint32_t)(((4 * device_cus(p->device)) + 7) / 8 * 8);
+    const bool f4w_park = path == 3 && par->ffun != GEPI_FFUNC_ENTROPY && !noalloc;
+    const size_t park_rec_bytes = (size_t)park_slots * F4W_PAIR_CAP * sizeof(PairRec);
     const bool fp4 = path == 3 || path == 4;
     if (noalloc && !fp4)
         return gepi_fail(GEPI_ERR_ARG, "gepi_scan_noalloc runs the fp4 count path only (path %d "
@@ -496,6 +505,10 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
             return rc;
         if (!fp4) grown(p->d_perm, p->perm_bytes, perm.size() * 4);
         if (!fp4 && (rc = grow(&p->d_perm, &p->perm_bytes, perm.size() * 4)) != GEPI_OK)
+            return rc;
+        const size_t park_bytes = f4w_park ? park_rec_bytes + (park_slots + 8) * sizeof(uint32_t) : 0;
+        if (park_bytes) grown(p->d_park, p->park_bytes, park_bytes);
+        if (park_bytes && (rc = grow(&p->d_park, &p->park_bytes, park_bytes)) != GEPI_OK)
             return rc;
         if (fp4) grown(p->d_cw, p->cw_bytes, cw.size() * sizeof(CompressWord));
         if (fp4 &&
@@ -604,6 +617,14 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     ca.st = b.state;
     ca.pend = b.pend;
     ca.pcap = pcap;
+    ca.park = f4w_park ? p->d_park : nullptr;
+    ca.park_busy = f4w_park ? reinterpret_cast<uint32_t *>(
+                                  reinterpret_cast<char *>(p->d_park) + park_rec_bytes)
+                            : nullptr;
+    ca.park_cap = F4W_PAIR_CAP;
+    ca.park_slots = park_slots;
+    if (f4w_park)
+        HIP_TRY(hipMemsetAsync(ca.park_busy, 0, (park_slots + 8) * sizeof(uint32_t), p->stream));
     ca.lfact = b.lfact;
     ExactArgs ea;
     ea.st = b.state;

@@ -70,7 +70,8 @@ typedef struct gepi_params {
     int32_t gpu_num;       /* -G: HIP device ordinal                           */
     int32_t ffun;          /* -f: gepi_ffunc                                   */
     int64_t topmost;       /* -N: results to keep                              */
-    int32_t count_path;    /* 0 = fastest (3 for entropy, 4 for Fisher),
+    int32_t count_path;    /* 0 = fastest (3 from 10 K steps of 128 samples,
+                            * else 4; 4 for Fisher in gepi_scan_noalloc),
                             * 1 = popcount (VALU), 2 = int8 MFMA, 3 = fp4
                             * (e2m1) MFMA on 32 x 32 wave tiles, 4 = fp4 MFMA
                             * on LDS-staged 64 x 32 tiles                    */

@@ -155,21 +155,23 @@ def test_fp4_super_step_residues(n0, n1, path):
     assert_same(got, ref, "fp4 super-step")
 
 
-def test_fisher_parked_pairs_overflow(monkeypatch):
-    """The LDS-staged fp4 kernel (path 4, Fisher's default) parks variant
-    pairs that pass the chi-square bound in LDS and finishes them (function bits, Fisher's exact
-    check) one per thread after the tile; with a loose threshold (large k,
-    two-tile bands) a tile flags more pairs than the list holds and the rest
-    are finished in the lane.  Together they must keep exactly the oracle's
-    top-k."""
-    print("case fisher parked pairs", flush=True)
+@pytest.mark.parametrize("path", [3, 4])
+def test_fisher_parked_pairs_overflow(monkeypatch, path):
+    """The fp4 kernels park variant pairs that pass Fisher's chi-square
+    bound (the LDS-staged kernel, path 4, in LDS; the wave tiles, path 3, in
+    a global region each workgroup claims) and finish them (function bits,
+    Fisher's exact check) one per thread after the tile; with a loose
+    threshold (large k, two-tile bands) a tile flags more pairs than the
+    list holds and the rest are finished in the lane.  Together they must
+    keep exactly the oracle's top-k."""
+    print("case fisher parked pairs path=%d" % path, flush=True)
     bits, valid, m0, m1 = random_panel(57, 300, 400, 0.02)
     k = 50000
     monkeypatch.setenv("GEPI_BAND_TILES", "2")
     got, st = ge.process_gpu_1(bits, valid, m0, m1, 400, topmost=k,
-                               ffun=ge.GEPI_FFUNC_FISHERPV)
+                               ffun=ge.GEPI_FFUNC_FISHERPV, count_path=path)
     monkeypatch.delenv("GEPI_BAND_TILES")
-    assert st["count_path"] == 4
+    assert st["count_path"] == path
     ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, 400), k, ffun=1)
     assert st["bands"] > 2
     assert_same(got, ref, "fisher-parked")
@@ -431,12 +433,11 @@ def test_cli_report_equals_oracle_report():
 
 
 @pytest.mark.parametrize("n0,n1,ffun,path", [(390, 310, 0, 4), (640, 640, 0, 3), (640, 512, 0, 4),
-                                              (1200, 1200, 1, 4), (1200, 1200, 0, 3)])
+                                              (640, 512, 1, 4), (1200, 1200, 1, 3), (1200, 1200, 0, 3)])
 def test_automatic_count_path(n0, n1, ffun, path):
-    """count_path 0 takes the fp4 wave tiles (3) for entropy scans of at
-    least 10 K steps of 128 samples (both classes, each rounded up) and the
-    LDS-staged fp4 kernel (4) otherwise and for Fisher; either way the
-    result equals the oracle's."""
+    """count_path 0 takes the fp4 wave tiles (3) for scans of at least 10 K
+    steps of 128 samples (both classes, each rounded up) and the LDS-staged
+    fp4 kernel (4) below that; either way the result equals the oracle's."""
     print("case auto path n0=%d n1=%d ffun=%d" % (n0, n1, ffun), flush=True)
     rng = np.random.default_rng(n0 + 3 * n1 + ffun)
     n = n0 + n1

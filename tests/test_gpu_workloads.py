@@ -169,3 +169,23 @@ def test_serve_mix_and_resident_rescans(mix):
     assert n_rescan >= 8
     print("serve: %d responses equal alone (%d re-scans), p50 %.1f ms p99 %.1f ms" % (
         n_req, n_rescan, st["p50_ms"], st["p99_ms"]), flush=True)
+
+
+def test_fisher_wave_tile_parking_item59():
+    """Fisher on the fp4 wave tiles parks flagged variant pairs in global
+    regions that workgroups claim and reuse within a launch.  Regions are
+    claimed per XCD: a region reused across XCDs read stale records when
+    another XCD's L2 wrote its dirty lines back over it, which this scan
+    (mix item 59: 30,286 x 4,966, 3,089 cases, top-100) showed on every run.
+    Three scans on path 3 must equal the LDS-staged kernel's (path 4)."""
+    print("case fisher wave-tile parking, mix item 59", flush=True)
+    sp = synth.mix_specs()[59]
+    p = synth.make_panel(spec=sp)
+    m0, m1 = synth.masks_from_pheno(p["pheno"])
+    with ge.Panel.from_bed(p["payload"], p["n_var"], p["n_samples"]) as h:
+        ref, st = h.scan(m0, m1, topmost=100, ffun=1, count_path=4)
+        assert st["count_path"] == 4
+        for _ in range(3):
+            got, st = h.scan(m0, m1, topmost=100, ffun=1)
+            assert st["count_path"] == 3
+            assert_same(got, ref, "item 59 path 3 vs path 4")
