@@ -159,7 +159,7 @@ def run_mix(args):
 
     def sequential():
         return [it["panel"].scan(it["mask_ctrl"], it["mask_case"], topmost=it["topmost"],
-                                 ffun=it["ffun"], count_path=3)[0] for it in items]
+                                 ffun=it["ffun"])[0] for it in items]
 
     for _ in range(args.warmup):
         sequential()
@@ -297,7 +297,7 @@ def run_serve(args):
             src = big if kind == "rescan" else p
             handles[pid] = ge.Panel.from_bed(src["payload"], src["n_var"], src["n_samples"])
     t0 = time.perf_counter()
-    alone = [handles[pid].scan(m0, m1, topmost=100, ffun=args.ffun, count_path=3)[0]
+    alone = [handles[pid].scan(m0, m1, topmost=100, ffun=args.ffun)[0]
              for pid, m0, m1, _, _, _ in reqs]
     seq_wall = time.perf_counter() - t0
     identical = all(np.array_equal(a, r[0]) and np.array_equal(a, q[0])
