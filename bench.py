@@ -201,7 +201,9 @@ def run_mix(args):
                                               for c in tune]} if args.autotune else None,
         "scan_latency_in_batch_ms": {"p50": lat[len(lat) // 2], "p99": lat[min(len(lat) - 1,
                                                                          int(0.99 * len(lat)))]},
-        "roofline": {"bound": "mfma", "kernel": "k_count_f4_batch",
+        "roofline": {"bound": "mfma", "kernel": " + ".join(
+                         {3: "k_count_f4w_batch", 4: "k_count_f4_batch"}[cp]
+                         for cp in sorted(set(st["count_path"] for _, st, _ in res))),
                      "achieved": ops / (k_ms / 1000.0) / 1e12, "peak": FP4_MFMA_PEAK_FLOPS / 1e12,
                      "unit": "TFLOP/s", "frac": ops / (k_ms / 1000.0) / FP4_MFMA_PEAK_FLOPS,
                      "traffic": None, "count_kernel_ms": k_ms},

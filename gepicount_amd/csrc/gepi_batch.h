@@ -1,11 +1,15 @@
 /*
  * gepi_batch.h -- kernels of the batched scan engine.  Many scans (panels
  * and phenotype vectors of different sizes) advance together in rounds; a
- * round is three launches whatever the number of scans:
+ * round is three kinds of launch whatever the number of scans (one count
+ * launch per objective and job shape present):
  *
  *   k_count_f4_batch  one band of fp4 count tiles per scan in one persistent
  *                     grid: workgroups take (scan, tile) slots from per-XCD
  *                     work counters through a descriptor table
+ *   k_count_f4w_batch the same for scans on wave tiles (Fisher's, and long
+ *                     entropy scans): one (scan, unit of two tiles) slot per
+ *                     workgroup
  *   k_exact_batch     exact scores of each scan's filter survivors
  *                     (blockIdx.y = scan)
  *   k_topk_wg         one 1024-thread workgroup per scan: radix select of
@@ -120,6 +124,27 @@ __global__ void __launch_bounds__(F4_THREADS, 2)
         const F4BatchOnceSrc src{d, map};
         count_f4_run<FFUN, HALF>(src);
     }
+}
+
+/* Wave-tile jobs (the engine's job shape 2): one unit, two 64 x 32-variant
+ * tiles as four 32 x 32 wave tiles (count_f4w_unit), per workgroup; slot
+ * blockIdx.x / 8 of XCD blockIdx.x % 8 as in F4BatchOnceSrc, whose ranges
+ * here count units (a descriptor's tile_begin and n_tiles still count
+ * tiles).  The scan's layout is the grouped 3-step one (k_class_f4<3,
+ * true>).  A slot past the round's units returns before any barrier. */
+template <int FFUN>
+__global__ void __launch_bounds__(F4W_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_count_f4w_batch(const BatchDesc *__restrict__ d, BatchMap map) {
+    __shared__ F4WShared sh;
+    const F4BatchOnceSrc src{d, map};
+    const F4Job j = src.grab(0);
+    if (!j.ok) return;
+    const CountArgs g = d[j.item].g;
+    const int64_t unit = j.tile - g.tile_begin;
+    if (g.f32)
+        count_f4w_unit<FFUN, true>(g, unit, sh);
+    else
+        count_f4w_unit<FFUN, false>(g, unit, sh);
 }
 
 /* p[0..n) = v (one workgroup of 64 threads per 64 words). */

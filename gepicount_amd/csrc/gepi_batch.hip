@@ -28,6 +28,14 @@ namespace {
 constexpr int64_t F4_TILE_EVALS = (int64_t)TM * TB4 * 20;
 constexpr int EXACT_BLOCKS_X = 64; /* blocks per scan of the batched exact pass */
 
+/* Job shape of an entropy scan with ks K steps under the default schedule:
+ * wave tiles from 48 K steps, whole tiles below.  Measured on the MI355X
+ * with 4 scans of 12,000 variants per batch against whole tiles in the
+ * persistent launch: 1.19-1.25x the time at 1,536-2,048 samples, 1.09-1.11x
+ * at 3,072-4,096, 0.97x at 6,144, 0.87x at 8,192.                        */
+constexpr int32_t BATCH_WAVE_MIN_KSTEPS = 48;
+int batch_shape_default(int32_t ks) { return ks >= BATCH_WAVE_MIN_KSTEPS ? 2 : 0; }
+
 double ms_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
@@ -78,6 +86,14 @@ void Engine::drop_graphs() {
 
 void Engine::launch_count(int g, bool persist, unsigned grid, const BatchMap &m) {
     const BatchDesc *d = d_bdesc_[g];
+    if (g >= 4) {
+        /* wave tiles: one unit per workgroup, no persistent form */
+        if (g == 4)
+            k_count_f4w_batch<GEPI_FFUNC_ENTROPY><<<grid, F4W_THREADS, 0, stream>>>(d, m);
+        else
+            k_count_f4w_batch<GEPI_FFUNC_FISHERPV><<<grid, F4W_THREADS, 0, stream>>>(d, m);
+        return;
+    }
     switch (g | (persist ? 4 : 0)) {
     case 0: k_count_f4_batch<GEPI_FFUNC_ENTROPY, false, false><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
     case 1: k_count_f4_batch<GEPI_FFUNC_FISHERPV, false, false><<<grid, F4_THREADS, 0, stream>>>(d, m); break;
@@ -98,7 +114,7 @@ int Engine::capture_graph(int part, int per_cu) {
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     if (part < NGRAPH_COUNT) {
         k_fill_u32<<<1, 64, 0, stream>>>(d_wq_, 16 * NGROUP, 0u);
-        for (int g = 0; g < NGROUP; g++) {
+        for (int g = 0; g < 4; g++) {
             if (!(part & (1 << g))) continue;
             /* the maps are laid out for cap scans; the slots past a round's
              * scans have empty ranges                                       */
@@ -155,6 +171,9 @@ void Engine::release() {
     hipFree(d_rep_);
     hipFree(d_wq_);
     d_wq_ = nullptr;
+    hipFree(d_park_);
+    d_park_ = nullptr;
+    park_slots_ = 0;
     d_edesc_ = nullptr;
     d_tdesc_ = nullptr;
     d_rep_ = nullptr;
@@ -267,7 +286,9 @@ int Engine::autotune(int32_t repeats) {
     const Trial trials[4] = {{2, 40000, 256}, {8, 28000, 1024}, {32, 18000, 4096}, {INT32_MAX, 12000, 8192}};
     const int per_cus[4] = {0, 1, 2, 3};
     const double bands[3] = {5e8, 2e9, 8e9};
-    const int tiles[2] = {0, 1};
+    /* job shapes: tiles, half tiles, wave tiles (not in captured rounds) */
+    const int n_tiles = use_graph_ ? 2 : 3;
+    const int tiles[3] = {0, 1, 2};
     repeats = std::max(1, std::min(repeats, 16));
     std::vector<gepi_tune_class> table;
     int rc = GEPI_OK;
@@ -307,22 +328,29 @@ int Engine::autotune(int32_t repeats) {
             masks.push_back(std::move(m1));
         }
         std::vector<ScanItem> items = scans_of(c);
-        gepi_tune_class best{t.max_k4, 2, 0, 0, 2e9, 0.0, 0.0};
+        /* the default schedule of the trial's scans (about half cases) */
+        const int dtile = batch_shape_default((t.n / 2 + 127) / 128 * 2);
+        gepi_tune_class best{t.max_k4, 2, dtile, 0, 2e9, 0.0, 0.0};
         double ms_best = INFINITY, ms_dflt = INFINITY, ms;
         /* warm-up (allocations, tables, graphs) under the default, then
          * once per job shape (the first launches of its kernels) */
-        for (int tl : tiles) {
-            tuning.assign(1, gepi_tune_class{INT32_MAX, 2, tl, 0, 2e9, 0.0, 0.0});
+        for (int ti = 0; ti < n_tiles; ti++) {
+            tuning.assign(1, gepi_tune_class{INT32_MAX, 2, tiles[ti], 0, 2e9, 0.0, 0.0});
             if (rc == GEPI_OK) rc = time_batch(items, &ms);
         }
         for (int r = 0; r < repeats && rc == GEPI_OK; r++)
-            for (int tl : tiles)
+            for (int ti = 0; ti < n_tiles; ti++)
                 for (int pc : per_cus)
                     for (double b : bands) {
+                        const int tl = tiles[ti];
+                        /* wave tiles have one launch form (a unit per
+                         * workgroup); their class keeps the default 2 for
+                         * the rounds it leads                                */
+                        if (tl == 2 && pc != 2) continue;
                         if (rc != GEPI_OK) break;
                         tuning.assign(1, gepi_tune_class{INT32_MAX, pc, tl, 0, b, 0.0, 0.0});
                         if ((rc = time_batch(items, &ms)) != GEPI_OK) break;
-                        if (tl == 0 && pc == 2 && b == 2e9) ms_dflt = std::min(ms_dflt, ms);
+                        if (tl == dtile && pc == 2 && b == 2e9) ms_dflt = std::min(ms_dflt, ms);
                         /* the best of each schedule over the repeats */
                         if (ms < ms_best) {
                             ms_best = ms;
@@ -335,7 +363,7 @@ int Engine::autotune(int32_t repeats) {
         if (ms_dflt <= 1.03 * ms_best) {
             /* within the trial's noise: keep the default */
             best.wg_per_cu = 2;
-            best.tile = 0;
+            best.tile = dtile;
             best.band_evals = 2e9;
             ms_best = ms_dflt;
         }
@@ -353,8 +381,9 @@ int Engine::autotune(int32_t repeats) {
             all.insert(all.end(), items.begin(), items.end());
         }
         bool differs = false;
-        for (const gepi_tune_class &c : table)
-            differs |= c.wg_per_cu != 2 || c.tile != 0 || c.band_evals != 2e9;
+        for (size_t c = 0; c < table.size(); c++)
+            differs |= table[c].wg_per_cu != 2 || table[c].band_evals != 2e9 ||
+                       table[c].tile != batch_shape_default((trials[c].n / 2 + 127) / 128 * 2);
         double ms_t = INFINITY, ms_d = INFINITY, ms;
         for (int r = 0; r < repeats && rc == GEPI_OK && differs; r++) {
             tuning = table;
@@ -365,11 +394,11 @@ int Engine::autotune(int32_t repeats) {
             ms_d = std::min(ms_d, ms);
         }
         if (rc == GEPI_OK && differs && !(ms_t < 0.99 * ms_d))
-            for (gepi_tune_class &c : table) {
-                c.wg_per_cu = 2;
-                c.tile = 0;
-                c.band_evals = 2e9;
-                c.ms_best = c.ms_default;
+            for (size_t c = 0; c < table.size(); c++) {
+                table[c].wg_per_cu = 2;
+                table[c].tile = batch_shape_default((trials[c].n / 2 + 127) / 128 * 2);
+                table[c].band_evals = 2e9;
+                table[c].ms_best = table[c].ms_default;
             }
         mixed_ms_tuned_ = ms_t;
         mixed_ms_default_ = ms_d;
@@ -466,7 +495,16 @@ int Engine::add(ScanItem *it) {
     const int64_t R = 2 * it->n_var;
     const int64_t total = 5 * (R * (R + 1) / 2);
     it->stats.evals = total;
-    it->stats.count_path = 3;
+    /* the job shape decides the class layout (wave tiles read the grouped
+     * one): Fisher scans run wave tiles, which park flagged pairs; entropy
+     * scans their class's tile (no table: batch_shape_default); captured
+     * rounds have no wave-tile launch                                      */
+    const int32_t ks = (it->n0 + 127) / 128 + (it->n1 + 127) / 128;
+    const gepi_tune_class *tc = tune_of(ks);
+    it->shape = tc ? tc->tile : batch_shape_default(ks);
+    if (it->ffun != GEPI_FFUNC_ENTROPY && !use_graph_) it->shape = 2;
+    if (it->shape == 2 && use_graph_) it->shape = 0;
+    it->stats.count_path = it->shape == 2 ? 3 : 4;
     it->stats.n_class0 = it->n0;
     it->stats.n_class1 = it->n1;
     if ((int64_t)it->n0 + it->n1 > GEPI_MAX_SAMPLES) {
@@ -532,14 +570,18 @@ int Engine::add(ScanItem *it) {
                            stream));
     if (it->n_k4 > 0) {
         const int64_t rows = it->n_var_pad * 3;
-        k_class_f4<3><<<(unsigned)((rows + 255) / 256), 256, 0, stream>>>(
-            p->d_nat, (int32_t)it->n_var, p->w64, it->d_cw, rows, f4_super3(it->ks0),
-            it->d_layout, nullptr);
+        const unsigned cgrid = (unsigned)((rows + 255) / 256);
+        if (it->shape == 2)
+            k_class_f4<3, true><<<cgrid, 256, 0, stream>>>(p->d_nat, (int32_t)it->n_var, p->w64,
+                                                           it->d_cw, rows, f4_super3(it->ks0),
+                                                           it->d_layout, nullptr);
+        else
+            k_class_f4<3><<<cgrid, 256, 0, stream>>>(p->d_nat, (int32_t)it->n_var, p->w64, it->d_cw,
+                                                     rows, f4_super3(it->ks0), it->d_layout,
+                                                     nullptr);
         HIP_TRY(hipGetLastError());
     }
-    const gepi_tune_class *tc = tune_of(it->n_k4);
     const double band_evals = tc ? tc->band_evals : 2.0e9;
-    it->half = tc && tc->tile == 1;
     it->max_band = std::max<int64_t>(64, (int64_t)(band_evals / (double)F4_TILE_EVALS));
     it->band = std::max<int64_t>(1, std::min<int64_t>(it->max_band, (2 * it->k) / F4_TILE_EVALS + 1));
     it->forced_band = 0;
@@ -589,12 +631,12 @@ int Engine::round(std::vector<ScanItem *> &finished) {
     /* count launch groups: objective (g & 1) x job shape (g >> 1) */
     std::vector<BatchDesc> bd[NGROUP];
     std::vector<ScanItem *> grp[NGROUP];
-    int64_t wg[NGROUP] = {0, 0, 0, 0};
+    int64_t wg[NGROUP] = {};
     std::vector<ExactDesc> ed(na);
     std::vector<TopkDesc> td(na);
     for (size_t i = 0; i < na; i++) {
         ScanItem *it = active_[i];
-        grp[(it->ffun == GEPI_FFUNC_ENTROPY ? 0 : 1) + (it->half ? 2 : 0)].push_back(it);
+        grp[(it->ffun == GEPI_FFUNC_ENTROPY ? 0 : 1) + 2 * it->shape].push_back(it);
         ExactDesc &e = ed[i];
         memset(&e, 0, sizeof e);
         e.e.st = it->d_state;
@@ -634,11 +676,29 @@ int Engine::round(std::vector<ScanItem *> &finished) {
     const int per_cu = f4_wg_per_cu(per_cu_dflt);
     /* captured rounds need a fixed grid: the persistent launch form */
     const bool graph = use_graph_ && per_cu > 0;
+    /* Fisher on wave tiles parks flagged pairs in regions of one pool */
+    uint32_t *park_busy = nullptr;
+    if (!grp[5].empty()) {
+        const int32_t slots = (int32_t)((4 * (int64_t)device_cus(device) + 7) / 8 * 8);
+        const size_t rec_bytes = (size_t)slots * F4W_PAIR_CAP * sizeof(PairRec);
+        if (park_slots_ != slots) {
+            hipFree(d_park_);
+            d_park_ = nullptr;
+            park_slots_ = 0;
+            HIP_TRY(hipMalloc(&d_park_, rec_bytes + (size_t)(slots + 8) * sizeof(uint32_t)));
+            park_slots_ = slots;
+        }
+        park_busy = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(d_park_) + rec_bytes);
+        HIP_TRY(hipMemsetAsync(park_busy, 0, (size_t)(slots + 8) * sizeof(uint32_t), stream));
+    }
     std::vector<int64_t> map[NGROUP];
     for (int f = 0; f < NGROUP; f++) {
         std::vector<ScanItem *> &g = grp[f];
-        /* half-tile jobs: two per tile, numbered 2 * tile + half */
-        const int64_t per_tile = f >= 2 ? 2 : 1;
+        /* jobs: tiles (shape 0), half tiles (1: two per tile, numbered
+         * 2 * tile + half) or units of two tiles (2: the descriptor still
+         * counts tiles)                                                     */
+        const int shape = f >> 1;
+        const int64_t per_tile = shape == 1 ? 2 : 1;
         std::stable_sort(g.begin(), g.end(),
                          [](const ScanItem *a, const ScanItem *b) { return a->n_k4 > b->n_k4; });
         const size_t n = g.size();
@@ -665,8 +725,15 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             d.g.pend = it->d_pend;
             d.g.pcap = it->pcap;
             d.g.lfact = tabs_.lfact;
+            if (f == 5) {
+                d.g.park = d_park_;
+                d.g.park_busy = park_busy;
+                d.g.park_cap = F4W_PAIR_CAP;
+                d.g.park_slots = (uint32_t)park_slots_;
+            }
             bd[f].push_back(d);
-            const int64_t b = it->band_used * per_tile, r = dealt & 7;
+            const int64_t b = shape == 2 ? (it->band_used + 1) / 2 : it->band_used * per_tile;
+            const int64_t r = dealt & 7;
             int64_t start = 0;
             for (int x = 0; x < 8; x++) {
                 const int64_t chunk = b / 8 + (((x - r) & 7) < (b & 7) ? 1 : 0);
@@ -708,7 +775,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             gx_per_cu_ = per_cu;
         }
         int mask = 0;
-        for (int f = 0; f < NGROUP; f++) mask |= wg[f] > 0 ? 1 << f : 0;
+        for (int f = 0; f < 4; f++) mask |= wg[f] > 0 ? 1 << f : 0; /* no wave tiles here */
         if (!gx_count_[mask] && (rc = capture_graph(mask, per_cu)) != GEPI_OK) return rc;
         if (!gx_finish_ && (rc = capture_graph(NGRAPH_COUNT, per_cu)) != GEPI_OK) return rc;
         /* the count graph clears the work counters first */
@@ -735,7 +802,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             int64_t xmax = 0;
             for (int x = 0; x < 8; x++) xmax = std::max<int64_t>(xmax, map[f][(size_t)x * (n + 1) + n]);
             unsigned grid = (unsigned)(8 * xmax);
-            if (per_cu > 0) {
+            if (per_cu > 0 && f < 4) {
                 wq_parity_[f] ^= 1;
                 grid = (unsigned)std::min<int64_t>(
                     std::max<int64_t>(8, (per_cu * (int64_t)device_cus(device)) / 8 * 8),
@@ -860,8 +927,8 @@ extern "C" int gepi_engine_set_tuning(gepi_engine *e, const gepi_tune_class *cla
             return gepi_fail(GEPI_ERR_ARG, "class %d: max_ksteps must increase", i);
         if (c.wg_per_cu < 0 || c.wg_per_cu > 16)
             return gepi_fail(GEPI_ERR_ARG, "class %d: wg_per_cu %d outside [0, 16]", i, c.wg_per_cu);
-        if (c.tile != 0 && c.tile != 1)
-            return gepi_fail(GEPI_ERR_ARG, "class %d: tile %d is neither 0 nor 1", i, c.tile);
+        if (c.tile < 0 || c.tile > 2)
+            return gepi_fail(GEPI_ERR_ARG, "class %d: tile %d is not 0, 1 or 2", i, c.tile);
         if (!(c.band_evals >= 1e6) || !std::isfinite(c.band_evals))
             return gepi_fail(GEPI_ERR_ARG, "class %d: band_evals below 1e6", i);
     }

@@ -974,22 +974,30 @@ __device__ __forceinline__ int64_t band_unit(int32_t n_units) {
  * pair in eight: with g.park those are parked in a global region the
  * workgroup claims (park_claim; the LDS holds the class-0 counts) and
  * finished one per thread after the filter (pair_drain); past g.park_cap,
- * or without regions, in the lane.                                       */
+ * or without regions, in the lane.
+ *
+ * The workgroup's unit (tiles tile_begin + 2 unit and 2 unit + 1 of the
+ * band) comes from the caller: k_count_f4w numbers units as band_tile
+ * numbers tiles, k_count_f4w_batch takes a (scan, unit) slot.  The
+ * workgroup-shared state lives in F4WShared.                             */
+struct F4WShared {
+    uint32_t c0lds[4][F4W_PARK * 64];
+    PendStage stage;
+    uint32_t n_park, park_slot;
+};
+
 template <int FFUN, bool F32>
-__global__ void __launch_bounds__(F4W_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_count_f4w(CountArgs g) {
-    __shared__ uint32_t c0lds[4][F4W_PARK * 64];
-    __shared__ PendStage stage;
+__device__ __forceinline__ void count_f4w_unit(const CountArgs &g, int64_t unit, F4WShared &sh) {
     constexpr bool PARK = FFUN == GEPI_FFUNC_FISHERPV;
-    __shared__ uint32_t n_park, park_slot;
+    PendStage &stage = sh.stage;
     stage_init(stage);
     if (PARK && threadIdx.x == 0) {
-        n_park = 0;
-        park_slot = g.park ? park_claim(g.park_busy, g.park_slots) : PARK_NONE;
+        sh.n_park = 0;
+        sh.park_slot = g.park ? park_claim(g.park_busy, g.park_slots) : PARK_NONE;
     }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int64_t tile = g.tile_begin + 2 * band_unit((g.n_tiles + 1) >> 1) + (wave >> 1);
+    const int64_t tile = g.tile_begin + 2 * unit + (wave >> 1);
     const bool live = tile < g.tile_begin + g.n_tiles;
     int32_t ta = 0, tb = 0;
     if (live) tile_of_f4(tile, g.n_var_pad / TM, ta, tb);
@@ -999,7 +1007,7 @@ k_count_f4w(CountArgs g) {
     const char *lay = reinterpret_cast<const char *>(g.planes_k);
     const int64_t ustride = (int64_t)g.n_var_pad * 192; /* bytes per super-step */
     const int loff = (lane >> 5) * 512 + (lane & 31) * 16;
-    uint32_t *P = c0lds[wave] + lane; /* dword d of this lane at P[64 d] */
+    uint32_t *P = sh.c0lds[wave] + lane; /* dword d of this lane at P[64 d] */
 
     /* c[cp][p]: A plane p, column h + 2cp; c[cp][3 + q]: B plane q */
     uint4 c[2][6];
@@ -1067,8 +1075,8 @@ k_count_f4w(CountArgs g) {
 
     const FilterConsts fc = filter_consts(g.st, g.n0, g.n1);
     __syncthreads(); /* the stage counter is cleared, the park region claimed */
-    const bool parks = PARK && park_slot != PARK_NONE;
-    PairListG pl{&n_park, parks ? g.park + (int64_t)park_slot * g.park_cap : nullptr,
+    const bool parks = PARK && sh.park_slot != PARK_NONE;
+    PairListG pl{&sh.n_park, parks ? g.park + (int64_t)sh.park_slot * g.park_cap : nullptr,
                  parks ? g.park_cap : 0u};
     const int32_t B = gB * 32 + (lane & 31);
     const int32_t Ab = gA * 32 + 4 * (lane >> 5);
@@ -1108,11 +1116,19 @@ k_count_f4w(CountArgs g) {
          * (pair_drain_l2); the region belongs to this XCD (park_claim). */
         f4_wait_stores();
         __syncthreads();
-        pair_drain_l2<FFUN>(g, stage, fc, pl.rec, n_park < pl.cap_ ? n_park : pl.cap_, pl.cap_);
+        pair_drain_l2<FFUN>(g, stage, fc, pl.rec, sh.n_park < pl.cap_ ? sh.n_park : pl.cap_,
+                            pl.cap_);
         __syncthreads(); /* the region is read: free it */
-        if (threadIdx.x == 0 && parks) atomicExch(&g.park_busy[park_slot], 0u);
+        if (threadIdx.x == 0 && parks) atomicExch(&g.park_busy[sh.park_slot], 0u);
     }
     stage_flush(g, stage);
+}
+
+template <int FFUN, bool F32>
+__global__ void __launch_bounds__(F4W_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_count_f4w(CountArgs g) {
+    __shared__ F4WShared sh;
+    count_f4w_unit<FFUN, F32>(g, band_unit((g.n_tiles + 1) >> 1), sh);
 }
 
 } // namespace gepi

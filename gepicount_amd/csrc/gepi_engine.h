@@ -27,6 +27,7 @@ struct BatchMap;
 struct ExactDesc;
 struct TopkDesc;
 struct ItemReport;
+struct PairRec;
 } // namespace gepi
 
 namespace gepi_host {
@@ -49,7 +50,9 @@ struct ScanItem {
     int64_t k = 0, n_var = 0, n_var_pad = 0, n_tiles = 0, t0 = 0;
     int64_t band = 1, band_used = 0, max_band = 1, forced_band = 0;
     int32_t n0 = 0, n1 = 0, n_k4 = 0, ks0 = 0;
-    bool half = false; /* count jobs are half tiles (its class's tile shape 1) */
+    /* count job shape (its class's tile): 0 tiles, 1 half tiles, 2 wave
+     * tiles on the grouped layout                                          */
+    int32_t shape = 0;
     unsigned long long cap = 0, pcap = 0, count = 0, exact_total = 0;
     uint4 *d_layout = nullptr;
     gepi::CompressWord *d_cw = nullptr;
@@ -108,7 +111,9 @@ class Engine {
      * gx_count_[part] (work-counter clear, then the count launches of the
      * groups in bit mask part: bit g = objective (g & 1: 0 entropy, 1
      * Fisher) and job shape (g >> 1: 0 whole tiles, 1 half tiles)), part
-     * NGRAPH_COUNT into gx_finish_ (exact + top-k launches).                */
+     * NGRAPH_COUNT into gx_finish_ (exact + top-k launches).  Wave-tile
+     * groups (4, 5) have no persistent form and are never captured: with
+     * graphs on, scans keep whole tiles instead.                          */
     int capture_graph(int part, int per_cu);
     /* One count launch of group g (as in capture_graph). */
     void launch_count(int g, bool persist, unsigned grid, const gepi::BatchMap &m);
@@ -119,17 +124,21 @@ class Engine {
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
     size_t in_use_ = 0;
-    /* count launch groups: objective x job shape */
-    static constexpr int NGROUP = 4, NGRAPH_COUNT = 16;
-    gepi::BatchDesc *d_bdesc_[NGROUP] = {nullptr, nullptr, nullptr, nullptr};
-    int64_t *d_map_[NGROUP] = {nullptr, nullptr, nullptr, nullptr};
+    /* count launch groups: objective (g & 1) x job shape (g >> 1) */
+    static constexpr int NGROUP = 6, NGRAPH_COUNT = 16;
+    gepi::BatchDesc *d_bdesc_[NGROUP] = {};
+    int64_t *d_map_[NGROUP] = {};
     gepi::ExactDesc *d_edesc_ = nullptr;
     gepi::TopkDesc *d_tdesc_ = nullptr;
     gepi::ItemReport *d_rep_ = nullptr;
     size_t desc_cap_ = 0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t *d_wq_ = nullptr; /* fp4 work counters: [group][2 sets][8] */
-    int32_t wq_parity_[NGROUP] = {0, 0, 0, 0};
+    int32_t wq_parity_[NGROUP] = {};
+    /* Fisher's parked variant pairs on wave tiles: park_slots_ regions of
+     * F4W_PAIR_CAP records, then park_slots_ + 8 busy / rotation words     */
+    gepi::PairRec *d_park_ = nullptr;
+    int32_t park_slots_ = 0;
     /* rounds as captured graphs (GEPI_ENGINE_GRAPH=1; else launched one by
      * one); valid for gx_cap_ descriptor slots and gx_per_cu_ workgroups
      * per CU                                                                 */

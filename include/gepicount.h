@@ -238,31 +238,41 @@ typedef struct gepi_tune_class {
     int32_t max_ksteps;
     int32_t wg_per_cu;   /* persistent count workgroups per CU (0: one tile
                             per workgroup)                                    */
-    int32_t tile;        /* count job shape: 0 = a 64 x 32-variant tile (four
-                            waves of 32 x 16), 1 = half of one, 32 x 32
-                            variants (four waves of 16 x 16)                  */
+    int32_t tile;        /* count job shape of the class's entropy scans:
+                            0 = a 64 x 32-variant tile (four waves of
+                            32 x 16), 1 = half of one, 32 x 32 variants (four
+                            waves of 16 x 16), 2 = two tiles as four 32 x 32
+                            wave tiles (count path 3's kernel, one unit per
+                            workgroup).  Fisher scans run wave tiles in every
+                            class (they park flagged pairs); captured rounds
+                            (GEPI_ENGINE_GRAPH=1) have no wave-tile launch and
+                            run whole tiles in its place.                    */
     int32_t reserved;
     double band_evals;   /* evaluations per scan and round                    */
     double ms_best;      /* autotune trial of the class: chosen schedule      */
-    double ms_default;   /* the same trial with 2 workgroups per CU, 2e9      */
+    double ms_default;   /* the same trial under the default schedule         */
 } gepi_tune_class;
 
 /* Autotuner: for each of 4 size classes (at most 2, 8, 32 and any number of
  * K steps), runs a synthetic batch of 4 scans of that class (40,000 variants
  * x 256 samples, 28,000 x 1,024, 18,000 x 4,096, 12,000 x 8,192; top-100)
  * under every candidate schedule (count job shape 64 x 32 or 32 x 32
- * variants x one tile per workgroup, or 1 or 2 persistent workgroups per CU
- * x 5e8, 2e9 or 8e9 evaluations per band), repeats times each, and keeps
- * the fastest (the default unless another is more than 3% faster).  The
- * engine uses the table from then on: a scan's job shape and band size by
- * its class, a round's launch form by the class with the most count work
- * in it.  Results never depend on the schedule.  Writes min(n_out, 4)
+ * variants x one tile per workgroup or 1, 2 or 3 persistent workgroups per
+ * CU, or wave tiles; x 5e8, 2e9 or 8e9 evaluations per band), repeats times
+ * each, and keeps the fastest (the default unless another is more than 3%
+ * faster).  The default schedule: 2 persistent workgroups per CU, 2e9
+ * evaluations per band, entropy scans of at least 48 K steps on wave tiles
+ * and smaller ones on 64 x 32 tiles.  The engine uses the table from then
+ * on: a scan's job shape and band size by its class, a round's launch form
+ * by the class with the most count work in it.  A table that differs from
+ * the default is kept only if it is also 1% faster on all 16 trial scans
+ * together.  Results never depend on the schedule.  Writes min(n_out, 4)
  * classes; *n_classes = 4.  On failure the engine is left with the default
  * schedule.                                                                */
 int gepi_engine_autotune(gepi_engine *engine, int32_t repeats, gepi_tune_class *out,
                          int32_t n_out, int32_t *n_classes);
 /* Installs a table (from an earlier autotune): 1-16 classes, max_ksteps
- * strictly increasing, wg_per_cu in [0, 16], tile 0 or 1, band_evals >= 1e6.
+ * strictly increasing, wg_per_cu in [0, 16], tile 0, 1 or 2, band_evals >= 1e6.
  * n = 0 restores the defaults.                                               */
 int gepi_engine_set_tuning(gepi_engine *engine, const gepi_tune_class *classes, int32_t n);
 /* Runs every item to completion.  Returns GEPI_OK when the batch ran (an

@@ -73,7 +73,8 @@ def test_batch_items_equal_alone_and_oracle(seed):
     assert bstats["rounds"] >= 1
     for i, (it, (got, st, status)) in enumerate(zip(items, res)):
         assert status == 0, (i, status)
-        assert st["count_path"] == 3
+        # the default schedule: Fisher on wave tiles, entropy below 48 K steps on whole tiles
+        assert st["count_path"] == (3 if it["ffun"] == 1 else 4), (i, st["count_path"])
         assert_same(got, alone[i], "item %d vs alone" % i)
         bits, valid, n = panels[it["panel_index"]]
         ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], n),
@@ -243,7 +244,8 @@ def test_batch_autotune_and_tables(monkeypatch):
         assert [c["max_ksteps"] for c in table] == [2, 8, 32, 2**31 - 1]
         for c in table:
             assert c["wg_per_cu"] in (0, 1, 2, 3) and c["band_evals"] in (5e8, 2e9, 8e9), c
-            assert c["tile"] in (0, 1), c
+            assert c["tile"] in (0, 1, 2), c
+            assert c["tile"] != 2 or c["wg_per_cu"] == 2, c
             assert 0 < c["ms_best"] <= c["ms_default"], c
         tuned, _ = run_batch(panels, items, eng)
         eng.set_tuning([dict(max_ksteps=8, wg_per_cu=1, tile=1, band_evals=1e6),
@@ -253,7 +255,7 @@ def test_batch_autotune_and_tables(monkeypatch):
         assert bs2["rounds"] >= bs0["rounds"]
         for bad in ([dict(max_ksteps=8, wg_per_cu=2, band_evals=2e9)] * 2,
                     [dict(max_ksteps=8, wg_per_cu=-1, band_evals=2e9)],
-                    [dict(max_ksteps=8, wg_per_cu=2, tile=2, band_evals=2e9)],
+                    [dict(max_ksteps=8, wg_per_cu=2, tile=3, band_evals=2e9)],
                     [dict(max_ksteps=8, wg_per_cu=2, band_evals=10.0)]):
             with pytest.raises(ge.GepiError):
                 eng.set_tuning(bad)
@@ -307,3 +309,64 @@ def test_batch_half_tile_jobs(per_cu, monkeypatch):
                                 it["topmost"], ffun=it["ffun"])
             assert r[i][2] == 0, (label, i, r[i][2])
             assert_same(r[i][0], ref, "%s item %d vs oracle" % (label, i))
+
+
+def ksteps(m0, m1):
+    n0 = int(sum(bin(int(w)).count("1") for w in np.asarray(m0, np.uint64)))
+    n1 = int(sum(bin(int(w)).count("1") for w in np.asarray(m1, np.uint64)))
+    return (n0 + 127) // 128 + (n1 + 127) // 128, n0, n1
+
+
+@pytest.mark.parametrize("variant", ["mixed", "overflow"])
+def test_batch_wave_tile_jobs(variant, monkeypatch):
+    """Scans whose class runs wave-tile jobs (two 64 x 32-variant tiles per
+    workgroup as four 32 x 32 wave tiles on the grouped layout, the count
+    kernel of count path 3) next to scans on half and whole tiles, both
+    objectives, fp32 and int32 filter counts (a class product n0 n1 above
+    2^24); every item equals the same scan alone and the oracle.  The
+    overflow variant forces 7-tile bands (odd: a unit's second tile lies past
+    the band) and a small candidate buffer."""
+    print("case batch wave tiles %s" % variant, flush=True)
+    monkeypatch.delenv("GEPI_F4_WG_PER_CU", raising=False)
+    monkeypatch.delenv("GEPI_ENGINE_GRAPH", raising=False)
+    panels, items = make_mix(71 if variant == "mixed" else 72, 8, vmax=500, nmax=3000)
+    rng = np.random.default_rng(73)
+    for n_var, n, frac in ((300, 1500, 0.3), (90, 8400, 0.5)):
+        codes = orc.random_codes(rng, n_var, n, 0.02)
+        bits, valid = orc.planes_from_codes(codes)
+        panels.append((bits, valid, n))
+        for ffun in (0, 1):
+            m0, m1 = orc.masks_from_classes(np.where(rng.random(n) < frac, 1, 0))
+            items.append(dict(panel_index=len(panels) - 1, mask_ctrl=m0, mask_case=m1,
+                              topmost=300 if ffun == 0 else 100, ffun=ffun))
+    if variant == "overflow":
+        items = [dict(it, topmost=20000 if it["ffun"] == 0 else 2000) for it in items]
+    table = [dict(max_ksteps=2, wg_per_cu=2, tile=1, band_evals=2e9),
+             dict(max_ksteps=4, wg_per_cu=2, tile=0, band_evals=2e9),
+             dict(max_ksteps=2**31 - 1, wg_per_cu=2, tile=2, band_evals=2e9)]
+    with ge.Engine() as eng:
+        eng.set_tuning(table)
+        if variant == "overflow":
+            monkeypatch.setenv("GEPI_CAND_CAP", "1")
+            monkeypatch.setenv("GEPI_BAND_TILES", "7")
+        res, bs, alone = run_mix(panels, items, eng)
+        for v in ("GEPI_CAND_CAP", "GEPI_BAND_TILES"):
+            monkeypatch.delenv(v, raising=False)
+    assert bs["items_failed"] == 0
+    n_wave = 0
+    for i, it in enumerate(items):
+        got, st, status = res[i]
+        assert status == 0, (i, status)
+        ks, n0, n1 = ksteps(it["mask_ctrl"], it["mask_case"])
+        wave = ks > 4 or it["ffun"] == 1          # Fisher runs wave tiles in any class
+        assert st["count_path"] == (3 if wave else 4), (i, ks, st["count_path"])
+        n_wave += ks > 4 and it["ffun"] == 0
+        assert_same(got, alone[i], "%s item %d (K steps %d) vs alone" % (variant, i, ks))
+        bits, valid, n = panels[it["panel_index"]]
+        ref = orc.gold_scan(orc.Panel(bits, valid, it["mask_ctrl"], it["mask_case"], n),
+                            it["topmost"], ffun=it["ffun"])
+        assert_same(got, ref, "%s item %d (K steps %d, n0 %d, n1 %d) vs oracle"
+                    % (variant, i, ks, n0, n1))
+    assert n_wave >= 2
+    if variant == "overflow":
+        assert sum(st["cand_overflows"] for _, st, _ in res) > 0
