@@ -31,8 +31,8 @@ constexpr int EXACT_BLOCKS_X = 64; /* blocks per scan of the batched exact pass 
 /* Job shape of an entropy scan with ks K steps under the default schedule:
  * wave tiles from 48 K steps, whole tiles below.  Measured on the MI355X
  * with 4 scans of 12,000 variants per batch against whole tiles in the
- * persistent launch: 1.19-1.25x the time at 1,536-2,048 samples, 1.09-1.11x
- * at 3,072-4,096, 0.97x at 6,144, 0.87x at 8,192.                        */
+ * persistent launch, count launches: 1.27-1.32x the time at 1,536-2,048
+ * samples, 1.10-1.14x at 3,072-4,096, 0.97x at 6,144, 0.85x at 8,192.     */
 constexpr int32_t BATCH_WAVE_MIN_KSTEPS = 48;
 int batch_shape_default(int32_t ks) { return ks >= BATCH_WAVE_MIN_KSTEPS ? 2 : 0; }
 
