@@ -970,11 +970,14 @@ __device__ __forceinline__ int64_t band_unit(int32_t n_units) {
  * LDS, two per dword; class 1 is
  * the final accumulator less the class-0 count.  Padding K steps of a
  * class's last super-step are zero in the layout and add nothing.  The
- * filter runs per lane on its 16 pairs.  Fisher flags about one variant
- * pair in eight: with g.park those are parked in a global region the
- * workgroup claims (park_claim; the LDS holds the class-0 counts) and
- * finished one per thread after the filter (pair_drain); past g.park_cap,
- * or without regions, in the lane.
+ * filter runs per lane on its 16 pairs.  A flagged variant pair (Fisher:
+ * about one in eight; entropy: one in a few hundred once the threshold
+ * has settled, yet in a third of a wave's filter steps some lane has one)
+ * is parked in a global region the workgroup claims (park_claim; the LDS
+ * holds the class-0 counts) and finished one per thread after the filter
+ * (pair_drain), so that no wave runs the function bits and pending-list
+ * pushes for one lane; past g.park_cap, or without regions (g.park null),
+ * in the lane.
  *
  * The workgroup's unit (tiles tile_begin + 2 unit and 2 unit + 1 of the
  * band) comes from the caller: k_count_f4w numbers units as band_tile
@@ -988,7 +991,7 @@ struct F4WShared {
 
 template <int FFUN, bool F32>
 __device__ __forceinline__ void count_f4w_unit(const CountArgs &g, int64_t unit, F4WShared &sh) {
-    constexpr bool PARK = FFUN == GEPI_FFUNC_FISHERPV;
+    constexpr bool PARK = true; /* both objectives park flagged pairs (g.park) */
     PendStage &stage = sh.stage;
     stage_init(stage);
     if (PARK && threadIdx.x == 0) {

@@ -410,11 +410,11 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         const bool fisher = par->ffun != GEPI_FFUNC_ENTROPY;
         path = ks >= F4W_MIN_KSTEPS && !(fisher && noalloc) ? 3 : 4;
     }
-    /* wave tiles, Fisher: parking regions, four per CU (at most two
-     * workgroups of the kernel are resident on a CU), an eighth of them per
-     * XCD */
+    /* wave tiles: parking regions for flagged variant pairs, four per CU
+     * (at most two workgroups of the kernel are resident on a CU), an
+     * eighth of them per XCD */
     const uint32_t park_slots = (uint32_t)(((4 * device_cus(p->device)) + 7) / 8 * 8);
-    const bool f4w_park = path == 3 && par->ffun != GEPI_FFUNC_ENTROPY && !noalloc;
+    const bool f4w_park = path == 3 && !noalloc;
     const size_t park_rec_bytes = (size_t)park_slots * F4W_PAIR_CAP * sizeof(PairRec);
     const bool fp4 = path == 3 || path == 4;
     if (noalloc && !fp4)
