@@ -1088,7 +1088,7 @@ __device__ __forceinline__ void count_f4w_unit(const CountArgs &g, int64_t unit,
         const int32_t A = Ab + 8 * (e >> 2) + (e & 3);
         if (!live || A >= g.n_var || B > A) continue;
         if constexpr (F32) {
-            float cn[2][3][3];
+            float cn[2][3][3], tot[3][3];
 #pragma unroll
             for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -1096,8 +1096,9 @@ __device__ __forceinline__ void count_f4w_unit(const CountArgs &g, int64_t unit,
                     const uint32_t w = P[((p * 3 + q) * 8 + e / 2) * 64];
                     cn[0][p][q] = (float)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
                     cn[1][p][q] = acc[p][q][e] - cn[0][p][q];
+                    tot[p][q] = acc[p][q][e];
                 }
-            filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, cn, &pl);
+            filter_pair<FFUN, false, PARK>(g, stage, fc, A, B, cn, &pl, tot);
         } else {
             int32_t cn[2][3][3];
 #pragma unroll

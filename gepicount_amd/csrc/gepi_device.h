@@ -437,7 +437,7 @@ struct PairList;
 template <int FFUN, bool ROLLED, bool DEFER = false, typename CT, typename PL = PairList>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                             int32_t A, int32_t B, const CT (&c)[2][3][3],
-                                            PL *pl = nullptr);
+                                            PL *pl = nullptr, const CT (*tot)[3] = nullptr);
 
 /* Variant pairs with a flagged row pair, parked in LDS: the lanes that
  * flag one are few and scattered over the waves, and finishing a pair (the
@@ -474,9 +474,11 @@ __device__ __forceinline__ uint32_t park_reserve(PairListG *pl) { return atomicA
 template <int FFUN, bool ROLLED, bool DEFER, typename CT, typename PL>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                             int32_t A, int32_t B, const CT (&c)[2][3][3],
-                                            PL *pl) {
+                                            PL *pl, const CT (*tot)[3]) {
+    /* tot: the products' totals over both classes, c[0] + c[1], when the
+     * caller holds them (the wave tiles' accumulators), else summed here */
     const CT y = c[0][2][2], z = c[1][2][2];
-    const CT m = y + z;
+    const CT m = tot ? tot[2][2] : y + z;
     const float mf = (float)m;
     float K = fc.kf;
     if (FFUN == GEPI_FFUNC_FISHERPV) {
@@ -505,11 +507,12 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
     }
     rt.bias = K == 0.0f ? -1.0f : 0.0f;
     auto dq = [&](CT q0, CT q1) { return dq_of(q0, q1, y, z); };
-    auto sq = [&](CT q0, CT q1) { return sq_of(q0, q1); };
+    /* S of product (p, q): both classes' count */
+    auto sq = [&](int p, int q) { return tot ? (float)tot[p][q] : sq_of(c[0][p][q], c[1][p][q]); };
     const CT Da0 = dq(c[0][0][2], c[1][0][2]), Da1 = dq(c[0][1][2], c[1][1][2]);
     const CT Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
-    const float Sa0 = sq(c[0][0][2], c[1][0][2]), Sa1 = sq(c[0][1][2], c[1][1][2]);
-    const float Sb0 = sq(c[0][2][0], c[1][2][0]), Sb1 = sq(c[0][2][1], c[1][2][1]);
+    const float Sa0 = sq(0, 2), Sa1 = sq(1, 2);
+    const float Sb0 = sq(2, 0), Sb1 = sq(2, 1);
     /* row pair rp = 2ia + ib flagged in bit rp */
     uint32_t rows = 0;
     if (ROLLED) {
@@ -519,7 +522,7 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
         float Sa = Sa0, Sb = Sb0, Sbo = Sb1;
 #pragma unroll 1
         for (int rp = 0; rp < 4; rp++) {
-            rows |= rt.any(dq(n0a, n1a), sq(n0a, n1a), Da, Sa, Db, Sb) ? 1u << rp : 0u;
+            rows |= rt.any(dq(n0a, n1a), sq_of(n0a, n1a), Da, Sa, Db, Sb) ? 1u << rp : 0u;
             n0a = n0b; n0b = n0c; n0c = n0d;
             n1a = n1b; n1b = n1c; n1c = n1d;
             if (rp == 1) { Da = Da1; Sa = Sa1; }
@@ -527,10 +530,10 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
             float u = Sb; Sb = Sbo; Sbo = u;
         }
     } else {
-        rows = (rt.any(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0) ? 1u : 0u) |
-               (rt.any(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) ? 2u : 0u) |
-               (rt.any(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) ? 4u : 0u) |
-               (rt.any(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) ? 8u : 0u);
+        rows = (rt.any(dq(c[0][0][0], c[1][0][0]), sq(0, 0), Da0, Sa0, Db0, Sb0) ? 1u : 0u) |
+               (rt.any(dq(c[0][0][1], c[1][0][1]), sq(0, 1), Da0, Sa0, Db1, Sb1) ? 2u : 0u) |
+               (rt.any(dq(c[0][1][0], c[1][1][0]), sq(1, 0), Da1, Sa1, Db0, Sb0) ? 4u : 0u) |
+               (rt.any(dq(c[0][1][1], c[1][1][1]), sq(1, 1), Da1, Sa1, Db1, Sb1) ? 8u : 0u);
     }
     if (A == B) rows &= ~2u; /* (2A, 2A+1) is not a row pair */
     if (DEFER && rows) {
@@ -552,13 +555,13 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
         /* rare: the function bits of the flagged row pairs */
         uint32_t mask = 0;
         if (rows & 1u)
-            mask |= rt.bits(dq(c[0][0][0], c[1][0][0]), sq(c[0][0][0], c[1][0][0]), Da0, Sa0, Db0, Sb0);
+            mask |= rt.bits(dq(c[0][0][0], c[1][0][0]), sq(0, 0), Da0, Sa0, Db0, Sb0);
         if (rows & 2u)
-            mask |= rt.bits(dq(c[0][0][1], c[1][0][1]), sq(c[0][0][1], c[1][0][1]), Da0, Sa0, Db1, Sb1) << 5;
+            mask |= rt.bits(dq(c[0][0][1], c[1][0][1]), sq(0, 1), Da0, Sa0, Db1, Sb1) << 5;
         if (rows & 4u)
-            mask |= rt.bits(dq(c[0][1][0], c[1][1][0]), sq(c[0][1][0], c[1][1][0]), Da1, Sa1, Db0, Sb0) << 10;
+            mask |= rt.bits(dq(c[0][1][0], c[1][1][0]), sq(1, 0), Da1, Sa1, Db0, Sb0) << 10;
         if (rows & 8u)
-            mask |= rt.bits(dq(c[0][1][1], c[1][1][1]), sq(c[0][1][1], c[1][1][1]), Da1, Sa1, Db1, Sb1) << 15;
+            mask |= rt.bits(dq(c[0][1][1], c[1][1][1]), sq(1, 1), Da1, Sa1, Db1, Sb1) << 15;
         int32_t ci[2][3][3];
 #pragma unroll
         for (int k = 0; k < 2; k++)
