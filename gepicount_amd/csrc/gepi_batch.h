@@ -141,10 +141,11 @@ k_count_f4w_batch(const BatchDesc *__restrict__ d, BatchMap map) {
     if (!j.ok) return;
     const CountArgs g = d[j.item].g;
     const int64_t unit = j.tile - g.tile_begin;
+    constexpr bool PARK = FFUN == GEPI_FFUNC_FISHERPV;
     if (g.f32)
-        count_f4w_unit<FFUN, true>(g, unit, sh);
+        count_f4w_unit<FFUN, true, PARK>(g, unit, sh);
     else
-        count_f4w_unit<FFUN, false>(g, unit, sh);
+        count_f4w_unit<FFUN, false, PARK>(g, unit, sh);
 }
 
 /* p[0..n) = v (one workgroup of 64 threads per 64 words). */

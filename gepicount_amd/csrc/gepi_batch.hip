@@ -676,9 +676,9 @@ int Engine::round(std::vector<ScanItem *> &finished) {
     const int per_cu = f4_wg_per_cu(per_cu_dflt);
     /* captured rounds need a fixed grid: the persistent launch form */
     const bool graph = use_graph_ && per_cu > 0;
-    /* scans on wave tiles park flagged pairs in regions of one pool */
+    /* Fisher scans on wave tiles park flagged pairs in regions of one pool */
     uint32_t *park_busy = nullptr;
-    if (!grp[4].empty() || !grp[5].empty()) {
+    if (!grp[5].empty()) {
         const int32_t slots = (int32_t)((4 * (int64_t)device_cus(device) + 7) / 8 * 8);
         const size_t rec_bytes = (size_t)slots * F4W_PAIR_CAP * sizeof(PairRec);
         if (park_slots_ != slots) {
@@ -725,7 +725,7 @@ int Engine::round(std::vector<ScanItem *> &finished) {
             d.g.pend = it->d_pend;
             d.g.pcap = it->pcap;
             d.g.lfact = tabs_.lfact;
-            if (shape == 2) {
+            if (f == 5) {
                 d.g.park = d_park_;
                 d.g.park_busy = park_busy;
                 d.g.park_cap = F4W_PAIR_CAP;

@@ -933,6 +933,9 @@ constexpr uint32_t F4W_PAIR_CAP = 1024;
  * at 20,000 variants, 0.99-1.09x its time up to 1,024 samples, 0.88-0.96x
  * from 1,536 on. */
 constexpr int32_t F4W_MIN_KSTEPS = 10;
+/* Entropy scans on wave tiles park flagged pairs from this many K steps of
+ * 128 samples (Fisher scans always do). */
+constexpr int32_t F4W_PARK_MIN_KSTEPS = 48;
 static_assert(GEPI_MAX_SAMPLES <= 65535, "a class count must fit 16 bits");
 
 /* Workgroup -> its two tiles (2u, 2u + 1 of the band), units numbered as
@@ -970,14 +973,16 @@ __device__ __forceinline__ int64_t band_unit(int32_t n_units) {
  * LDS, two per dword; class 1 is
  * the final accumulator less the class-0 count.  Padding K steps of a
  * class's last super-step are zero in the layout and add nothing.  The
- * filter runs per lane on its 16 pairs.  A flagged variant pair (Fisher:
- * about one in eight; entropy: one in a few hundred once the threshold
- * has settled, yet in a third of a wave's filter steps some lane has one)
- * is parked in a global region the workgroup claims (park_claim; the LDS
- * holds the class-0 counts) and finished one per thread after the filter
- * (pair_drain), so that no wave runs the function bits and pending-list
- * pushes for one lane; past g.park_cap, or without regions (g.park null),
- * in the lane.
+ * filter runs per lane on its 16 pairs.  PARK: a flagged variant pair
+ * (Fisher: about one in eight; entropy: one in a few hundred once the
+ * threshold has settled, yet in a third of a wave's filter steps some lane
+ * has one) is parked in a global region the workgroup claims (park_claim;
+ * the LDS holds the class-0 counts) and finished one per thread after the
+ * filter (pair_drain), so that no wave runs the function bits and
+ * pending-list pushes for one lane; past g.park_cap, or without regions
+ * (g.park null), in the lane.  The claim, the store wait and two barriers
+ * cost every workgroup a few microseconds, repaid only by long K loops
+ * (F4W_PARK_MIN_KSTEPS); without PARK every lane finishes its own pairs.
  *
  * The workgroup's unit (tiles tile_begin + 2 unit and 2 unit + 1 of the
  * band) comes from the caller: k_count_f4w numbers units as band_tile
@@ -989,9 +994,8 @@ struct F4WShared {
     uint32_t n_park, park_slot;
 };
 
-template <int FFUN, bool F32>
+template <int FFUN, bool F32, bool PARK>
 __device__ __forceinline__ void count_f4w_unit(const CountArgs &g, int64_t unit, F4WShared &sh) {
-    constexpr bool PARK = true; /* both objectives park flagged pairs (g.park) */
     PendStage &stage = sh.stage;
     stage_init(stage);
     if (PARK && threadIdx.x == 0) {
@@ -1128,11 +1132,11 @@ __device__ __forceinline__ void count_f4w_unit(const CountArgs &g, int64_t unit,
     stage_flush(g, stage);
 }
 
-template <int FFUN, bool F32>
+template <int FFUN, bool F32, bool PARK>
 __global__ void __launch_bounds__(F4W_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_count_f4w(CountArgs g) {
     __shared__ F4WShared sh;
-    count_f4w_unit<FFUN, F32>(g, band_unit((g.n_tiles + 1) >> 1), sh);
+    count_f4w_unit<FFUN, F32, PARK>(g, band_unit((g.n_tiles + 1) >> 1), sh);
 }
 
 } // namespace gepi

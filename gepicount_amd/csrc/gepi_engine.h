@@ -135,7 +135,7 @@ class Engine {
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     uint32_t *d_wq_ = nullptr; /* fp4 work counters: [group][2 sets][8] */
     int32_t wq_parity_[NGROUP] = {};
-    /* parked variant pairs of scans on wave tiles: park_slots_ regions of
+    /* Fisher's parked variant pairs on wave tiles: park_slots_ regions of
      * F4W_PAIR_CAP records, then park_slots_ + 8 busy / rotation words     */
     gepi::PairRec *d_park_ = nullptr;
     int32_t park_slots_ = 0;

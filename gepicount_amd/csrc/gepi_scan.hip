@@ -658,6 +658,16 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     /* fp4: the filter runs on the fp32 accumulators when every product of a
      * class-0 and a class-1 count is exact in float */
     const bool f32 = ca.f32 != 0;
+    /* Wave tiles: Fisher scans park flagged variant pairs (its bound flags
+     * about one pair in eight), entropy scans from F4W_PARK_MIN_KSTEPS K
+     * steps.  Parking costs every workgroup a fixed few microseconds (the
+     * region claim by atomics, the store wait, two barriers), which a
+     * short K loop does not repay: measured, 64 uneven scans of at most
+     * about 40 K steps one at a time took 237 ms parking in every band
+     * and 197 ms in none (249 ms parking only in bands that let fewer
+     * than 0.02 evaluations per variant pair through); the large scan
+     * (64 K steps) 126.0 ms parking and 129.2 ms not.                     */
+    const bool park = f4w_park && (!entropy || (n0 + 127) / 128 + (n1 + 127) / 128 >= F4W_PARK_MIN_KSTEPS);
     while (t0 < n_tiles) {
         const int64_t t1 = std::min<int64_t>(n_tiles, t0 + band);
         ca.tile_begin = t0;
@@ -667,14 +677,18 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         HIP_TRY(hipEventRecord(p->ev0, p->stream));
         const unsigned grid = path == 3 ? (unsigned)((t1 - t0 + 1) / 2) : (unsigned)(t1 - t0);
         if (path == 3) {
-            if (entropy && f32)
-                k_count_f4w<GEPI_FFUNC_ENTROPY, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+            if (entropy && f32 && park)
+                k_count_f4w<GEPI_FFUNC_ENTROPY, true, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+            else if (entropy && f32)
+                k_count_f4w<GEPI_FFUNC_ENTROPY, true, false><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+            else if (entropy && park)
+                k_count_f4w<GEPI_FFUNC_ENTROPY, false, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
             else if (entropy)
-                k_count_f4w<GEPI_FFUNC_ENTROPY, false><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+                k_count_f4w<GEPI_FFUNC_ENTROPY, false, false><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
             else if (f32)
-                k_count_f4w<GEPI_FFUNC_FISHERPV, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+                k_count_f4w<GEPI_FFUNC_FISHERPV, true, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
             else
-                k_count_f4w<GEPI_FFUNC_FISHERPV, false><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
+                k_count_f4w<GEPI_FFUNC_FISHERPV, false, true><<<grid, F4W_THREADS, 0, p->stream>>>(ca);
         } else if (path == 4) {
             if (entropy && f32)
                 k_count_f4<GEPI_FFUNC_ENTROPY, true><<<grid, F4_THREADS, 0, p->stream>>>(ca);
