@@ -413,7 +413,10 @@ class Engine:
         """Runs the batch autotuner (gepi_engine_autotune) and returns its
         table: one dict per size class (max_ksteps, wg_per_cu, tile,
         band_evals, ms_best, ms_default), ascending max_ksteps; tile 0 is the
-        64 x 32-variant count job, 1 the 32 x 32 half tile."""
+        64 x 32-variant count job, 1 the 32 x 32 half tile, 2 wave tiles (two
+        64 x 32 tiles as four 32 x 32 wave tiles, the kernel of count path 3).
+        tile schedules entropy scans; Fisher scans run wave tiles in every
+        class (not in captured-graph rounds)."""
         out = (TuneClass * 16)()
         n = ctypes.c_int32()
         _check(lib().gepi_engine_autotune(self._h, int(repeats), out, 16, ctypes.byref(n)))
