@@ -289,6 +289,11 @@ int Engine::autotune(int32_t repeats) {
     /* job shapes: tiles, half tiles, wave tiles (not in captured rounds) */
     const int n_tiles = use_graph_ ? 2 : 3;
     const int tiles[3] = {0, 1, 2};
+    /* the default schedule's shape for a class's trial scans (about half
+     * cases); captured rounds run whole tiles in place of wave tiles     */
+    auto dflt_tile = [&](int c) {
+        return use_graph_ ? 0 : batch_shape_default((trials[c].n / 2 + 127) / 128 * 2);
+    };
     repeats = std::max(1, std::min(repeats, 16));
     std::vector<gepi_tune_class> table;
     int rc = GEPI_OK;
@@ -328,8 +333,7 @@ int Engine::autotune(int32_t repeats) {
             masks.push_back(std::move(m1));
         }
         std::vector<ScanItem> items = scans_of(c);
-        /* the default schedule of the trial's scans (about half cases) */
-        const int dtile = batch_shape_default((t.n / 2 + 127) / 128 * 2);
+        const int dtile = dflt_tile(c);
         gepi_tune_class best{t.max_k4, 2, dtile, 0, 2e9, 0.0, 0.0};
         double ms_best = INFINITY, ms_dflt = INFINITY, ms;
         /* warm-up (allocations, tables, graphs) under the default, then
@@ -383,7 +387,7 @@ int Engine::autotune(int32_t repeats) {
         bool differs = false;
         for (size_t c = 0; c < table.size(); c++)
             differs |= table[c].wg_per_cu != 2 || table[c].band_evals != 2e9 ||
-                       table[c].tile != batch_shape_default((trials[c].n / 2 + 127) / 128 * 2);
+                       table[c].tile != dflt_tile((int)c);
         double ms_t = INFINITY, ms_d = INFINITY, ms;
         for (int r = 0; r < repeats && rc == GEPI_OK && differs; r++) {
             tuning = table;
@@ -396,7 +400,7 @@ int Engine::autotune(int32_t repeats) {
         if (rc == GEPI_OK && differs && !(ms_t < 0.99 * ms_d))
             for (size_t c = 0; c < table.size(); c++) {
                 table[c].wg_per_cu = 2;
-                table[c].tile = batch_shape_default((trials[c].n / 2 + 127) / 128 * 2);
+                table[c].tile = dflt_tile((int)c);
                 table[c].band_evals = 2e9;
                 table[c].ms_best = table[c].ms_default;
             }

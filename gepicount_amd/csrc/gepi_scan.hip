@@ -410,11 +410,21 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
         const bool fisher = par->ffun != GEPI_FFUNC_ENTROPY;
         path = ks >= F4W_MIN_KSTEPS && !(fisher && noalloc) ? 3 : 4;
     }
-    /* wave tiles: parking regions for flagged variant pairs, four per CU
-     * (at most two workgroups of the kernel are resident on a CU), an
-     * eighth of them per XCD */
+    /* Wave tiles: Fisher scans park flagged variant pairs (its bound flags
+     * about one pair in eight), entropy scans from F4W_PARK_MIN_KSTEPS K
+     * steps, in regions of a global pool, four per CU (at most two
+     * workgroups of the kernel are resident on a CU), an eighth of them per
+     * XCD.  Parking costs every workgroup a fixed few microseconds (the
+     * region claim by atomics, the store wait, two barriers), which a
+     * short K loop does not repay: measured, 64 uneven scans of at most
+     * about 40 K steps one at a time took 237 ms parking in every band
+     * and 197 ms in none (249 ms parking only in bands that let fewer
+     * than 0.02 evaluations per variant pair through); the large scan
+     * (64 K steps) 126.0 ms parking and 129.2 ms not.                     */
     const uint32_t park_slots = (uint32_t)(((4 * device_cus(p->device)) + 7) / 8 * 8);
-    const bool f4w_park = path == 3 && !noalloc;
+    const bool f4w_park =
+        path == 3 && !noalloc &&
+        (par->ffun != GEPI_FFUNC_ENTROPY || (n0 + 127) / 128 + (n1 + 127) / 128 >= F4W_PARK_MIN_KSTEPS);
     const size_t park_rec_bytes = (size_t)park_slots * F4W_PAIR_CAP * sizeof(PairRec);
     const bool fp4 = path == 3 || path == 4;
     if (noalloc && !fp4)
@@ -658,16 +668,8 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
     /* fp4: the filter runs on the fp32 accumulators when every product of a
      * class-0 and a class-1 count is exact in float */
     const bool f32 = ca.f32 != 0;
-    /* Wave tiles: Fisher scans park flagged variant pairs (its bound flags
-     * about one pair in eight), entropy scans from F4W_PARK_MIN_KSTEPS K
-     * steps.  Parking costs every workgroup a fixed few microseconds (the
-     * region claim by atomics, the store wait, two barriers), which a
-     * short K loop does not repay: measured, 64 uneven scans of at most
-     * about 40 K steps one at a time took 237 ms parking in every band
-     * and 197 ms in none (249 ms parking only in bands that let fewer
-     * than 0.02 evaluations per variant pair through); the large scan
-     * (64 K steps) 126.0 ms parking and 129.2 ms not.                     */
-    const bool park = f4w_park && (!entropy || (n0 + 127) / 128 + (n1 + 127) / 128 >= F4W_PARK_MIN_KSTEPS);
+    /* wave tiles: the kernel built with parking where the scan parks */
+    const bool park = f4w_park;
     while (t0 < n_tiles) {
         const int64_t t1 = std::min<int64_t>(n_tiles, t0 + band);
         ca.tile_begin = t0;
