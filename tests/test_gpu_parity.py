@@ -177,6 +177,32 @@ def test_fisher_parked_pairs_overflow(monkeypatch, path):
     assert_same(got, ref, "fisher-parked")
 
 
+@pytest.mark.parametrize("seed,n_var,n,k,band", [
+    (61, 150, 6600, 3000, None),     # float filter counts
+    (62, 97, 6600, 40000, "3"),      # loose threshold, odd 3-tile bands: regions overflow
+    (63, 161, 12500, 2000, None)])   # n0 n1 > 2^24: int32 filter counts
+def test_entropy_parking_long_k(monkeypatch, seed, n_var, n, k, band):
+    """Entropy scans of at least 48 K steps of 128 samples run the wave-tile
+    kernel built to park flagged variant pairs (a global region per
+    workgroup, finished one per thread after the filter; past the region's
+    capacity, in the lane).  Small panels (diagonal and edge tiles), a loose
+    threshold with odd bands, and class sizes past the float filter's range
+    must keep exactly the oracle's top-k."""
+    print("case entropy parking seed=%d n=%d" % (seed, n), flush=True)
+    bits, valid, m0, m1 = random_panel(seed, n_var, n, 0.02)
+    n0 = int(sum(bin(int(w)).count("1") for w in m0))
+    n1 = int(sum(bin(int(w)).count("1") for w in m1))
+    assert (n0 + 127) // 128 + (n1 + 127) // 128 >= 48
+    if band:
+        monkeypatch.setenv("GEPI_BAND_TILES", band)
+    got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k)
+    if band:
+        monkeypatch.delenv("GEPI_BAND_TILES")
+    assert st["count_path"] == 3
+    ref = orc.gold_scan(orc.Panel(bits, valid, m0, m1, n), k)
+    assert_same(got, ref, "entropy parking")
+
+
 def test_kat_test_fixture_top5():
     d = ge.read_plink(os.path.join(DATA, "test"))
     got, _ = ge.process_gpu_1(d.bit_data, d.valid_data, d.mask_ctrl, d.mask_case, d.n_samples,
