@@ -910,7 +910,8 @@ __device__ __forceinline__ void pair_drain_l2(const CountArgs &g, PendStage &ps,
             for (int p = 0; p < 3; p++)
 #pragma unroll
                 for (int q = 0; q < 3; q++) c[k][p][q] = r.c[k][p][q];
-        filter_pair<FFUN, true>(g, ps, fc, r.A, r.B, c);
+        filter_pair<FFUN, true>(g, ps, fc, r.A, r.B, c, (PairList *)nullptr,
+                                (const int32_t(*)[3]) nullptr, r.pad);
     }
 }
 

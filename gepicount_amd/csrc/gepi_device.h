@@ -437,7 +437,8 @@ struct PairList;
 template <int FFUN, bool ROLLED, bool DEFER = false, typename CT, typename PL = PairList>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                             int32_t A, int32_t B, const CT (&c)[2][3][3],
-                                            PL *pl = nullptr, const CT (*tot)[3] = nullptr);
+                                            PL *pl = nullptr, const CT (*tot)[3] = nullptr,
+                                            uint32_t rows_known = 0);
 
 /* Variant pairs with a flagged row pair, parked in LDS: the lanes that
  * flag one are few and scattered over the waves, and finishing a pair (the
@@ -474,7 +475,7 @@ __device__ __forceinline__ uint32_t park_reserve(PairListG *pl) { return atomicA
 template <int FFUN, bool ROLLED, bool DEFER, typename CT, typename PL>
 __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, const FilterConsts &fc,
                                             int32_t A, int32_t B, const CT (&c)[2][3][3],
-                                            PL *pl, const CT (*tot)[3]) {
+                                            PL *pl, const CT (*tot)[3], uint32_t rows_known) {
     /* tot: the products' totals over both classes, c[0] + c[1], when the
      * caller holds them (the wave tiles' accumulators), else summed here */
     const CT y = c[0][2][2], z = c[1][2][2];
@@ -513,9 +514,12 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
     const CT Db0 = dq(c[0][2][0], c[1][2][0]), Db1 = dq(c[0][2][1], c[1][2][1]);
     const float Sa0 = sq(0, 2), Sa1 = sq(1, 2);
     const float Sb0 = sq(2, 0), Sb1 = sq(2, 1);
-    /* row pair rp = 2ia + ib flagged in bit rp */
+    /* row pair rp = 2ia + ib flagged in bit rp; rows_known: flagged before
+     * the pair was parked (that test is as conservative as this one) */
     uint32_t rows = 0;
-    if (ROLLED) {
+    if (rows_known) {
+        rows = rows_known;
+    } else if (ROLLED) {
         CT n0a = c[0][0][0], n0b = c[0][0][1], n0c = c[0][1][0], n0d = c[0][1][1];
         CT n1a = c[1][0][0], n1b = c[1][0][1], n1c = c[1][1][0], n1d = c[1][1][1];
         CT Da = Da0, Db = Db0, Dbo = Db1;
@@ -548,6 +552,7 @@ __device__ __forceinline__ void filter_pair(const CountArgs &g, PendStage &ps, c
                     for (int q = 0; q < 3; q++) r.c[k][p][q] = (uint16_t)c[k][p][q];
             r.A = A;
             r.B = B;
+            r.pad = rows; /* the drain starts from these rows */
             return;
         }
     }
@@ -590,7 +595,8 @@ __device__ __forceinline__ void pair_drain(const CountArgs &g, PendStage &ps, co
             for (int p = 0; p < 3; p++)
 #pragma unroll
                 for (int q = 0; q < 3; q++) c[k][p][q] = r.c[k][p][q];
-        filter_pair<FFUN, true>(g, ps, fc, r.A, r.B, c);
+        filter_pair<FFUN, true>(g, ps, fc, r.A, r.B, c, (PairList *)nullptr,
+                                (const int32_t(*)[3]) nullptr, r.pad);
     }
 }
 
