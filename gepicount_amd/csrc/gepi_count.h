@@ -935,8 +935,11 @@ constexpr uint32_t F4W_PAIR_CAP = 1024;
  * from 1,536 on. */
 constexpr int32_t F4W_MIN_KSTEPS = 10;
 /* Entropy scans on wave tiles park flagged pairs from this many K steps of
- * 128 samples (Fisher scans always do). */
-constexpr int32_t F4W_PARK_MIN_KSTEPS = 48;
+ * 128 samples (Fisher scans always do).  Measured, entropy top-100 at 20,000
+ * variants and top-1,000 at 40,000: parking took 1.13-1.38x the time at 24-40
+ * K steps, 0.96x and 1.10x at 48, 0.96x and 0.96x at 64 (the large scan,
+ * 65 K steps: 0.98x). */
+constexpr int32_t F4W_PARK_MIN_KSTEPS = 64;
 static_assert(GEPI_MAX_SAMPLES <= 65535, "a class count must fit 16 bits");
 
 /* Workgroup -> its two tiles (2u, 2u + 1 of the band), units numbered as

@@ -420,7 +420,7 @@ int run_scan(gepi_panel *p, const uint64_t *mask_ctrl, const uint64_t *mask_case
      * about 40 K steps one at a time took 237 ms parking in every band
      * and 197 ms in none (249 ms parking only in bands that let fewer
      * than 0.02 evaluations per variant pair through); the large scan
-     * (64 K steps) 126.0 ms parking and 129.2 ms not.                     */
+     * (65 K steps) 126.0 ms parking and 129.2 ms not.                     */
     const uint32_t park_slots = (uint32_t)(((4 * device_cus(p->device)) + 7) / 8 * 8);
     const bool f4w_park =
         path == 3 && !noalloc &&

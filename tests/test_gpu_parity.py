@@ -177,22 +177,22 @@ def test_fisher_parked_pairs_overflow(monkeypatch, path):
     assert_same(got, ref, "fisher-parked")
 
 
-@pytest.mark.parametrize("seed,n_var,n,k,band", [
-    (61, 150, 6600, 3000, None),     # float filter counts
-    (62, 97, 6600, 40000, "3"),      # loose threshold, odd 3-tile bands: regions overflow
-    (63, 161, 12500, 2000, None)])   # n0 n1 > 2^24: int32 filter counts
-def test_entropy_parking_long_k(monkeypatch, seed, n_var, n, k, band):
-    """Entropy scans of at least 48 K steps of 128 samples run the wave-tile
+@pytest.mark.parametrize("seed,n_var,n,k,band,cls_p", [
+    (61, 150, 8500, 3000, None, (0.82, 0.14, 0.04)),    # float filter counts
+    (62, 97, 8500, 40000, "3", (0.82, 0.14, 0.04)),     # loose threshold, odd 3-tile bands
+    (63, 161, 12500, 2000, None, (0.5, 0.45, 0.05))])   # n0 n1 > 2^24: int32 filter counts
+def test_entropy_parking_long_k(monkeypatch, seed, n_var, n, k, band, cls_p):
+    """Entropy scans of at least 64 K steps of 128 samples run the wave-tile
     kernel built to park flagged variant pairs (a global region per
     workgroup, finished one per thread after the filter; past the region's
     capacity, in the lane).  Small panels (diagonal and edge tiles), a loose
     threshold with odd bands, and class sizes past the float filter's range
     must keep exactly the oracle's top-k."""
     print("case entropy parking seed=%d n=%d" % (seed, n), flush=True)
-    bits, valid, m0, m1 = random_panel(seed, n_var, n, 0.02)
+    bits, valid, m0, m1 = random_panel(seed, n_var, n, 0.02, cls_p)
     n0 = int(sum(bin(int(w)).count("1") for w in m0))
     n1 = int(sum(bin(int(w)).count("1") for w in m1))
-    assert (n0 + 127) // 128 + (n1 + 127) // 128 >= 48
+    assert (n0 + 127) // 128 + (n1 + 127) // 128 >= 64
     if band:
         monkeypatch.setenv("GEPI_BAND_TILES", band)
     got, st = ge.process_gpu_1(bits, valid, m0, m1, n, topmost=k)
